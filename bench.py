@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Benchmark of the SAC perturbation-MHD hot path on one MI355X.
+
+Workload: the stratified 3-D perturbation on a static background at 256^3
+(rho_b=exp(-z/H), isothermal, uniform vertical B with beta=1 at the base,
+hydrostatic gravity on axis 1, Gaussian velocity pulse plus seeded noise,
+fixed/periodic/periodic boundaries, 4th-order central differences + RK4 +
+hyperdiffusion), FP64.  A step is one full RK4 time step (four stages, each
+with its getboundary) including the per-step dt computation; inputs are
+resident in HBM when the timed region starts.
+
+Metric: Mcell-updates/s per RK stage = N_mesh * steps * 4 / wall time.
+Roofline: the dominant kernel (the fused stage kernel), algorithmic bytes =
+296 B per mesh cell per stage (4*n_evolve*8 + n_background*8 for 8 evolving
+and 5 background FP64 fields) x N_mesh per launch, divided by the kernel's
+average duration measured with HIP events on the library's stream.
+cpu_baseline: the C oracle (a scalar CPU restatement of the reference) on a
+32^3 sample of the same workload, one core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STAGE = {"fp64": 296}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="mesh cells per axis")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(budget_s: float):
+    """Time the C oracle on a bounded sample of the same workload."""
+    from oracle import oracle as om
+    from sac_amd import problems
+    from tests.helpers import make_oracle
+    om.build()
+    n = 32
+    prob = problems.stratified(n=n)
+    o = make_oracle(prob, om)
+    o.step(prob.courantpar)  # first step (no dt_visc history) untimed
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        o.step(prob.courantpar)
+        steps += 1
+    el = time.perf_counter() - t0
+    mcu = n ** 3 * steps * 4 / el / 1e6
+    return {"value": round(mcu, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"stratified {n}^3, {steps} RK4 steps after one warm-up step, "
+                      f"{el:.1f} s, C oracle (gcc -O2) on one host core"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    import numpy as np
+    from sac_amd import device, problems
+
+    n = args.n
+    prob = problems.stratified(n=n)
+    dev = device.Device(prob)
+    dev.upload(prob.w)
+    dev.getboundary()
+    del prob.w
+    for _ in range(args.warmup):
+        dev.step(0.3)
+    dev.synchronize()
+    dev.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.step(0.3)
+    dev.synchronize()
+    el = time.perf_counter() - t0
+    k_ms, k_n = dev.timing_get(device.K_STAGE)
+    r_ms, r_n = dev.timing_get(device.K_RATIO)
+    g_ms, g_n = dev.timing_get(device.K_GHOST)
+    dev.timing(False)
+    w = dev.download()
+    finite = bool(np.isfinite(w).all())
+    dev.close()
+
+    ncell = n ** 3
+    stages = args.steps * 4
+    value = ncell * stages / el / 1e6
+    avg_stage_kernel_s = (k_ms / max(k_n, 1)) / 1e3
+    alg_bytes = BYTES_PER_CELL_STAGE["fp64"] * ncell
+    achieved = alg_bytes / avg_stage_kernel_s / 1e9
+    stage_wall_s = el / stages
+    out = {
+        "metric": "Mcell-updates/s per RK stage at 256^3 (stratified perturbation MHD)",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"stratified 3-D perturbation MHD, {n}^3 mesh, FP64, RK4 + "
+                               "hyperdiffusion + gravity, fixed/periodic/periodic",
+                   "mesh": [n, n, n], "precision": "fp64", "typeadvance": "fourstep"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "k_stage<3,3>",
+                     "avg_ms": round(k_ms / max(k_n, 1), 4),
+                     "launches": k_n},
+        "stage_breakdown_ms": {"k_stage": round(k_ms / max(k_n, 1), 4),
+                               "k_ratio": round(r_ms / max(r_n, 1), 4),
+                               "k_ghost": round(g_ms / max(g_n, 1), 4),
+                               "stage_wall": round(stage_wall_s * 1e3, 4)},
+        "stage_roofline_frac": round(alg_bytes / stage_wall_s / 1e9 / HBM_PEAK_GBS, 4),
+        "finite": finite,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,145 @@
+/*
+ * sac_amd.h -- C ABI of the MI355X (gfx950) implementation of SAC's
+ * explicit perturbation-MHD update.
+ *
+ * The library owns device copies of the state; the caller owns the host
+ * array.  Host arrays use the reference layout: the Fortran array
+ * w(ixG1,ixG2,ixG3,nw), column-major, index 1 fastest, two ghost layers per
+ * active axis (src/vacdef.t:18-23), variables in the order of
+ * src/vacpar.t.mhd:9-13 (rho, m1..m_ndir, e, b1..b_ndir, eb, rhob,
+ * bg1..bg_ndir).
+ *
+ * Every function returns 0 on success or a positive code (SAC_E_*); the
+ * message of the last failure on the calling thread is returned by
+ * sac_dev_last_error().  There is no CPU fallback: without a usable GPU,
+ * sac_dev_init fails with SAC_E_HIP.
+ *
+ * Plain C: pointers and sizes only, all arguments by value or pointer, so a
+ * Fortran host binds them with ISO_C_BINDING interfaces (host/sac_bind.f90).
+ */
+#ifndef SAC_AMD_H
+#define SAC_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAC_OK 0
+#define SAC_E_ARG 1         /* invalid argument */
+#define SAC_E_UNSUPPORTED 2 /* configuration outside this library's scope */
+#define SAC_E_HIP 3         /* HIP runtime error (no device, launch failure) */
+#define SAC_E_NOMEM 4       /* device allocation failed */
+
+/* boundary codes for typeB (src/vacgrid.t:257-464) */
+#define SAC_BC_CONT 1
+#define SAC_BC_FIXED 2
+#define SAC_BC_PERIODIC 3
+#define SAC_BC_SYMM 4
+#define SAC_BC_SYMM0 5
+#define SAC_BC_ASYMM 6
+
+/* typeadvance codes (src/vac.t:406-480) */
+#define SAC_ADV_ONESTEP 1
+#define SAC_ADV_FOURSTEP 4
+
+/* precision of the evolving fields */
+#define SAC_PREC_FP64 0
+
+/* device registers addressed by the slot-level entry points */
+#define SAC_SLOT_W 0  /* the state w */
+#define SAC_SLOT_S1 1 /* stage registers */
+#define SAC_SLOT_S2 2
+#define SAC_SLOT_ACC 3
+
+/* kernels reported by sac_dev_timing_get */
+#define SAC_K_STAGE 0
+#define SAC_K_RATIO 1
+#define SAC_K_GHOST 2
+#define SAC_K_CMAX 3
+
+typedef void *sac_handle;
+
+/*
+ * Set up one problem on the current device.  Replaces the grid, boundary and
+ * method set-up of readparameters/boundsetup/gridsetup1
+ * (src/vacio.t:5-530, src/vacgrid.t:9-160,556-661) for the 'cd4' path.
+ *   ndim, ndir      dimensionality (1<=ndim<=ndir<=3)
+ *   nx[ndim]        mesh cells per axis (>= 6)
+ *   nw              number of variables, must be 4+3*ndir
+ *   iws[niws]       1-based variables with typefull='cd4' (iw_full,
+ *                   src/vacio.t:249-262); must be the evolving set 1..2+2*ndir
+ *   eqpar[neqpar]   equation + user parameters (gamma_=1, eta_=2,
+ *                   gravity d at grav0_+d, hyperdiffusion switch at nu_);
+ *                   eta must be 0
+ *   dx              cell widths per axis after gridsetup1, concatenated:
+ *                   dx_1[ixG1], dx_2[ixG2], dx_3[ixG3] (tensor grid)
+ *   typeB[iw*2*ndim+iB]  boundary code per variable and boundary region
+ *                   (regions ordered lower/upper per axis, src/vacgrid.t:53-69)
+ *   fullgridini     1 if the ini file carried ghost cells
+ *   typeadvance     SAC_ADV_FOURSTEP or SAC_ADV_ONESTEP
+ *   precision       SAC_PREC_FP64
+ */
+int sac_dev_init(int ndim, int ndir, const int *nx, int nw, const int *iws, int niws,
+                 const double *eqpar, int neqpar, int grav0_, int nu_, const double *dx,
+                 const int *typeB, int fullgridini, int typeadvance, int precision,
+                 sac_handle *out);
+
+/* Release all device memory of a handle. */
+int sac_dev_free(sac_handle h);
+
+/* Copy the state w (reference layout, nw variables, whole grid) to / from
+ * the device.  The static background (variables eb..bg) is taken from the
+ * upload and returned unchanged by the download. */
+int sac_dev_upload(sac_handle h, const double *w);
+int sac_dev_download(sac_handle h, double *w);
+
+/* Slot-level copies of the evolving variables (1..2+2*ndir) for stage-level
+ * parity checks; the background part of w is ignored on upload and filled
+ * from the device background on download. */
+int sac_dev_upload_slot(sac_handle h, int slot, const double *w);
+int sac_dev_download_slot(sac_handle h, int slot, double *w);
+
+/* getboundary(t,1,nw,1,ndim,w) on the state (src/vacgrid.t:195-512); the
+ * first call stores the 'fixed' boundary values as the reference does. */
+int sac_dev_getboundary(sac_handle h);
+
+/* getdt_courant (src/vac.t:641-699): dt = courantpar/max(cmax_d/dx_d) over
+ * the mesh; also returns courantmax.  The max comes from the reduction fused
+ * into the last stage that produced the state. */
+int sac_dev_getdt_courant(sac_handle h, double courantpar, double *dt, double *courantmax);
+
+/* getdt_special of the gravity+hyperdiffusion user module
+ * (src/vacusr.f:1403-1421): dt = min(dt, getdt_visc, getdt_grav) with
+ * getdt_visc (src/vacusr.viscosity.t:509-541) using the maxviscoef gathered
+ * since the previous call, which it then resets, and getdt_grav
+ * (src/vacusr.gravity.t:97-138). */
+int sac_dev_getdt_special(sac_handle h, double *dt);
+
+/* advance(iw_full,w) (src/vac.t:224-281) with dimsplit=F, sourcesplit=F:
+ * one full Runge-Kutta step of the state including every getboundary. */
+int sac_dev_advance(sac_handle h, double dt);
+
+/* advect1 with method 'cd4' (src/vac.t:490-549) on device registers:
+ * slot w <- w + qdt*R(slot wCT), then getboundary(w). */
+int sac_dev_advect1(sac_handle h, double qdt, int wct_slot, int w_slot);
+
+/* maxviscoef (src/vacdef.t:176) accumulated by setnu since the last
+ * getdt_special. */
+int sac_dev_get_maxviscoef(sac_handle h, double *v);
+int sac_dev_set_maxviscoef(sac_handle h, double v);
+
+/* Kernel timing with HIP events on the library's stream. */
+int sac_dev_timing(sac_handle h, int enable);
+int sac_dev_timing_get(sac_handle h, int kernel, double *total_ms, long long *count);
+int sac_dev_synchronize(sac_handle h);
+
+/* Bytes of device memory held by the handle. */
+int sac_dev_memory(sac_handle h, long long *bytes);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *sac_dev_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,631 @@
+// sac_abi.hip -- C ABI (include/sac_amd.h) over the HIP kernels: device
+// buffers, the Runge-Kutta stage sequence, ghost fills and time-step
+// reductions.  One handle = one problem on one device; all work is issued
+// on the handle's own stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sac_amd.h"
+#include "sac_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) return fail(SAC_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kSlots = 4;
+constexpr double kSmallDouble = 1.0e-99;  // src/vacdef.t:50
+constexpr double kBigDouble = 1.0e99;
+
+struct TimedLaunch {
+  int kernel;
+  hipEvent_t a, b;
+};
+
+struct Dev {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sac::GridDesc g{};
+  int ndim = 0, ndir = 0, nw = 0, ne = 0, nbg = 0, npairs = 0;
+  int nx[3] = {1, 1, 1};
+  long long ncell_grid = 0;  // G1*G2*G3 (unpadded, host layout)
+  int typeadvance = SAC_ADV_FOURSTEP;
+  int fullgridini = 0;
+  double gamma = 0.0;
+  double grav[3] = {0, 0, 0};
+  int grav_on[3] = {0, 0, 0};
+  int visc_on = 0;
+  double dtgrav = kBigDouble;
+  double dxmin[3] = {0, 0, 0};
+  bool any_fixed = false;
+  bool initialized = false;  // getboundary's 'initialized' (src/vacgrid.t:209)
+  // device buffers
+  double* reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+  double* bg = nullptr;
+  double* ratio = nullptr;
+  double* snap = nullptr;   // 'fixed' values of the evolving fields
+  double* bgsnap = nullptr;
+  double* dx[3] = {nullptr, nullptr, nullptr};
+  int* bc_e = nullptr;      // [ne][6]
+  int* bc_b = nullptr;      // [nbg][6]
+  unsigned long long* red = nullptr;  // [kSlots+1][kRedSlots]
+  bool red_valid[kSlots] = {false, false, false, false};
+  std::vector<double> stage;  // host staging
+  long long bytes = 0;
+  bool timing = false;
+  std::vector<TimedLaunch> timed;
+  double t_ms[4] = {0, 0, 0, 0};
+  long long t_n[4] = {0, 0, 0, 0};
+};
+
+long long fieldsz(const Dev& d) { return d.g.fs; }
+
+int dalloc(Dev& d, double** p, long long nelem) {
+  hipError_t e = hipMalloc((void**)p, sizeof(double) * (size_t)nelem);
+  if (e != hipSuccess) return fail(SAC_E_NOMEM, "hipMalloc of %lld doubles failed: %s", nelem,
+                                   hipGetErrorString(e));
+  d.bytes += (long long)sizeof(double) * nelem;
+  e = hipMemset(*p, 0, sizeof(double) * (size_t)nelem);
+  if (e != hipSuccess) return fail(SAC_E_HIP, "hipMemset: %s", hipGetErrorString(e));
+  return 0;
+}
+
+unsigned long long* red_of(Dev& d, int slot) { return d.red + (size_t)slot * sac::kRedSlots; }
+unsigned long long* red_visc(Dev& d) { return d.red + (size_t)kSlots * sac::kRedSlots; }
+
+// event helpers
+void t_begin(Dev& d, int k, hipEvent_t* a) {
+  if (!d.timing) return;
+  hipEventCreate(a);
+  hipEventRecord(*a, d.stream);
+  (void)k;
+}
+void t_end(Dev& d, int k, hipEvent_t a) {
+  if (!d.timing) return;
+  hipEvent_t b;
+  hipEventCreate(&b);
+  hipEventRecord(b, d.stream);
+  d.timed.push_back({k, a, b});
+}
+void t_collect(Dev& d) {
+  for (auto& t : d.timed) {
+    hipEventSynchronize(t.b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t.a, t.b);
+    d.t_ms[t.kernel] += ms;
+    d.t_n[t.kernel] += 1;
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  d.timed.clear();
+}
+
+void ghost_fill_evolving(Dev& d, int slot) {
+  sac::GhostArgs a{};
+  a.g = d.g;
+  a.f = d.reg[slot];
+  a.bc = d.bc_e;
+  a.nvar = d.ne;
+  a.snap = d.initialized ? d.snap : nullptr;
+  a.keep_fixed = (!d.initialized && d.fullgridini) ? 1 : 0;
+  hipEvent_t e0{};
+  t_begin(d, SAC_K_GHOST, &e0);
+  sac::launch_ghost(a, d.stream);
+  t_end(d, SAC_K_GHOST, e0);
+}
+
+void cmax_reduce(Dev& d, int slot) {
+  sac::launch_zero_u64(red_of(d, slot), sac::kRedSlots, d.stream);
+  sac::CmaxArgs a{};
+  a.g = d.g;
+  a.w = d.reg[slot];
+  a.bg = d.bg;
+  for (int k = 0; k < 3; k++) a.dx[k] = d.dx[k];
+  a.gamma = d.gamma;
+  a.red_out = red_of(d, slot);
+  hipEvent_t e0{};
+  t_begin(d, SAC_K_CMAX, &e0);
+  sac::launch_cmax(a, d.stream);
+  t_end(d, SAC_K_CMAX, e0);
+  d.red_valid[slot] = true;
+}
+
+// one stage: out = base + qa*R(wct) ; acc_out = acc_in + qb*R ; ghosts of out
+void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_out, double qb) {
+  if (!d.red_valid[wct]) cmax_reduce(d, wct);
+  if (d.visc_on) {
+    sac::RatioArgs r{};
+    r.g = d.g;
+    r.wct = d.reg[wct];
+    r.bg = d.bg;
+    r.ratio = d.ratio;
+    hipEvent_t e0{};
+    t_begin(d, SAC_K_RATIO, &e0);
+    sac::launch_ratio(r, d.stream);
+    t_end(d, SAC_K_RATIO, e0);
+  }
+  sac::launch_zero_u64(red_of(d, out), sac::kRedSlots, d.stream);
+  sac::StageArgs s{};
+  s.g = d.g;
+  s.wct = d.reg[wct];
+  s.bg = d.bg;
+  s.base = d.reg[base];
+  s.out = d.reg[out];
+  s.acc_in = acc_in >= 0 ? d.reg[acc_in] : nullptr;
+  s.acc_out = acc_out >= 0 ? d.reg[acc_out] : nullptr;
+  s.ratio = d.ratio;
+  for (int k = 0; k < 3; k++) s.dx[k] = d.dx[k];
+  s.qa = qa;
+  s.qb = qb;
+  s.gamma = d.gamma;
+  for (int k = 0; k < 3; k++) {
+    s.grav[k] = d.grav[k];
+    s.grav_on[k] = d.grav_on[k];
+  }
+  s.visc_on = d.visc_on;
+  s.ctot_src = red_of(d, wct);
+  s.red_out = red_of(d, out);
+  s.red_visc = red_visc(d);
+  s.reduce_out = 1;
+  hipEvent_t e0{};
+  t_begin(d, SAC_K_STAGE, &e0);
+  sac::launch_stage(s, d.stream);
+  t_end(d, SAC_K_STAGE, e0);
+  d.red_valid[out] = true;
+  if (acc_out >= 0) d.red_valid[acc_out] = false;
+  ghost_fill_evolving(d, out);
+}
+
+int check_handle(sac_handle h) {
+  if (!h) return fail(SAC_E_ARG, "null handle");
+  return 0;
+}
+
+int copy_in(Dev& d, double* dst, const double* src, int nfield) {
+  // src: nfield fields of G1*G2*G3 doubles; dst: padded layout
+  const size_t w = sizeof(double) * (size_t)d.g.G[0];
+  const size_t spitch = w;
+  const size_t dpitch = sizeof(double) * (size_t)d.g.st[1];
+  const size_t rows = (size_t)d.g.G[1] * (size_t)d.g.G[2];
+  for (int f = 0; f < nfield; f++) {
+    HIPCHK(hipMemcpy2DAsync(dst + (size_t)f * d.g.fs, dpitch, src + (size_t)f * d.ncell_grid,
+                            spitch, w, rows, hipMemcpyHostToDevice, d.stream));
+  }
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return 0;
+}
+
+int copy_out(Dev& d, double* dst, const double* src, int nfield) {
+  const size_t w = sizeof(double) * (size_t)d.g.G[0];
+  const size_t dpitch = w;
+  const size_t spitch = sizeof(double) * (size_t)d.g.st[1];
+  const size_t rows = (size_t)d.g.G[1] * (size_t)d.g.G[2];
+  for (int f = 0; f < nfield; f++) {
+    HIPCHK(hipMemcpy2DAsync(dst + (size_t)f * d.ncell_grid, dpitch, src + (size_t)f * d.g.fs,
+                            spitch, w, rows, hipMemcpyDeviceToHost, d.stream));
+  }
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return 0;
+}
+
+double red_host(Dev& d, int slot, int k) {
+  unsigned long long v = 0;
+  hipMemcpyAsync(&v, red_of(d, slot) + k, sizeof(v), hipMemcpyDeviceToHost, d.stream);
+  hipStreamSynchronize(d.stream);
+  double x;
+  memcpy(&x, &v, sizeof(x));
+  return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sac_dev_last_error(void) { return g_err.c_str(); }
+
+int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int niws,
+                 const double* eqpar, int neqpar, int grav0_, int nu_, const double* dx,
+                 const int* typeB, int fullgridini, int typeadvance, int precision,
+                 sac_handle* out) {
+  g_err.clear();
+  if (!out) return fail(SAC_E_ARG, "out is null");
+  *out = nullptr;
+  if (ndim < 1 || ndim > 3 || ndir < ndim || ndir > 3)
+    return fail(SAC_E_ARG, "need 1<=ndim<=ndir<=3 (ndim=%d ndir=%d)", ndim, ndir);
+  if (ndim == 2 && ndir == 2) {
+  } else if (!(ndim == 1 || (ndim == 2 && ndir == 3) || (ndim == 3 && ndir == 3))) {
+    return fail(SAC_E_UNSUPPORTED, "ndim=%d ndir=%d not built", ndim, ndir);
+  }
+  if (ndim == 1 && ndir < 1) return fail(SAC_E_ARG, "ndir");
+  if (!nx || !eqpar || !dx || !typeB || !iws) return fail(SAC_E_ARG, "null array argument");
+  if (nw != 4 + 3 * ndir) return fail(SAC_E_ARG, "nw=%d, expected %d", nw, 4 + 3 * ndir);
+  const int ne = 2 + 2 * ndir;
+  if (niws != ne) return fail(SAC_E_UNSUPPORTED, "iws must list the %d evolving variables", ne);
+  for (int i = 0; i < niws; i++)
+    if (iws[i] != i + 1) return fail(SAC_E_UNSUPPORTED, "iws must be 1..%d in order", ne);
+  if (neqpar < 2) return fail(SAC_E_ARG, "neqpar=%d < 2", neqpar);
+  if (eqpar[0] <= 0.0) return fail(SAC_E_ARG, "gamma must be > 0");
+  if (eqpar[1] != 0.0) return fail(SAC_E_UNSUPPORTED, "resistive MHD (eta /= 0) is not built");
+  if (typeadvance != SAC_ADV_FOURSTEP && typeadvance != SAC_ADV_ONESTEP)
+    return fail(SAC_E_UNSUPPORTED, "typeadvance code %d not built", typeadvance);
+  if (precision != SAC_PREC_FP64) return fail(SAC_E_UNSUPPORTED, "precision %d not built", precision);
+  for (int d = 0; d < ndim; d++)
+    if (nx[d] < 6) return fail(SAC_E_ARG, "nx[%d]=%d < 6", d, nx[d]);
+  for (int iw = 0; iw < nw; iw++)
+    for (int iB = 0; iB < 2 * ndim; iB++) {
+      int c = typeB[iw * 2 * ndim + iB];
+      if (c < SAC_BC_CONT || c > SAC_BC_ASYMM)
+        return fail(SAC_E_UNSUPPORTED, "boundary code %d (var %d, region %d)", c, iw + 1, iB + 1);
+    }
+  // periodic pairing must be consistent (src/vacgrid.t:109-120)
+  for (int d = 0; d < ndim; d++) {
+    int lo = typeB[0 * 2 * ndim + 2 * d], hi = typeB[0 * 2 * ndim + 2 * d + 1];
+    if ((lo == SAC_BC_PERIODIC) != (hi == SAC_BC_PERIODIC))
+      return fail(SAC_E_ARG, "Error in BoundSetup: No periodic pair (axis %d)", d + 1);
+  }
+
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev < 1)
+    return fail(SAC_E_HIP, "no HIP device available (%s)", hipGetErrorString(he));
+
+  Dev* d = new Dev();
+  HIPCHK(hipGetDevice(&d->device));
+  d->ndim = ndim;
+  d->ndir = ndir;
+  d->nw = nw;
+  d->ne = ne;
+  d->nbg = 2 + ndir;
+  d->npairs = ndim * (2 * ndim + 1);
+  d->typeadvance = typeadvance;
+  d->fullgridini = fullgridini;
+  d->gamma = eqpar[0];
+  sac::GridDesc& g = d->g;
+  g.ndim = ndim;
+  g.ndir = ndir;
+  long long ncg = 1;
+  for (int k = 0; k < 3; k++) {
+    if (k < ndim) {
+      d->nx[k] = nx[k];
+      g.G[k] = nx[k] + 4;
+      g.Mlo[k] = 2;
+      g.Mhi[k] = nx[k] + 1;
+    } else {
+      d->nx[k] = 1;
+      g.G[k] = 1;
+      g.Mlo[k] = 0;
+      g.Mhi[k] = 0;
+    }
+    ncg *= g.G[k];
+  }
+  d->ncell_grid = ncg;
+  const int pitch = (g.G[0] + 15) / 16 * 16;  // 128-byte rows
+  g.st[0] = 1;
+  g.st[1] = pitch;
+  g.st[2] = (long long)pitch * g.G[1];
+  g.fs = (long long)pitch * g.G[1] * g.G[2];
+  for (int k = 0; k < ndim; k++) {
+    g.per_lo[k] = typeB[0 * 2 * ndim + 2 * k] == SAC_BC_PERIODIC;
+    g.per_hi[k] = typeB[0 * 2 * ndim + 2 * k + 1] == SAC_BC_PERIODIC;
+  }
+  // user parameters: gravity at grav0_+d, switch at nu_ (1-based)
+  for (int k = 0; k < ndim; k++) {
+    int slot = grav0_ + k + 1;
+    double gk = (slot >= 1 && slot <= neqpar) ? eqpar[slot - 1] : 0.0;
+    d->grav[k] = gk;
+    d->grav_on[k] = std::fabs(gk) > kSmallDouble;
+  }
+  double nu = (nu_ >= 1 && nu_ <= neqpar) ? eqpar[nu_ - 1] : 0.0;
+  d->visc_on = std::fabs(nu) > kSmallDouble;
+  for (int iw = 0; iw < nw; iw++)
+    for (int iB = 0; iB < 2 * ndim; iB++)
+      if (typeB[iw * 2 * ndim + iB] == SAC_BC_FIXED) d->any_fixed = true;
+
+  int rc = 0;
+  auto bail = [&](int code) {
+    sac_dev_free(d);
+    return code;
+  };
+  if (hipStreamCreate(&d->stream) != hipSuccess) return bail(fail(SAC_E_HIP, "hipStreamCreate"));
+  for (int s = 0; s < kSlots; s++)
+    if ((rc = dalloc(*d, &d->reg[s], (long long)ne * g.fs))) return bail(rc);
+  if ((rc = dalloc(*d, &d->bg, (long long)d->nbg * g.fs))) return bail(rc);
+  if (d->visc_on && (rc = dalloc(*d, &d->ratio, (long long)d->npairs * g.fs))) return bail(rc);
+  if (d->any_fixed) {
+    if ((rc = dalloc(*d, &d->snap, (long long)ne * g.fs))) return bail(rc);
+    if ((rc = dalloc(*d, &d->bgsnap, (long long)d->nbg * g.fs))) return bail(rc);
+  }
+  // dx per axis, dxmin per axis over the mesh
+  long long off = 0;
+  for (int k = 0; k < 3; k++) {
+    int n = g.G[k];
+    if ((rc = dalloc(*d, &d->dx[k], n))) return bail(rc);
+    if (k < ndim) {
+      if (hipMemcpy(d->dx[k], dx + off, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SAC_E_HIP, "dx upload"));
+      double mn = kBigDouble;
+      for (int i = g.Mlo[k]; i <= g.Mhi[k]; i++) {
+        if (!(dx[off + i] > 0.0)) return bail(fail(SAC_E_ARG, "dx must be > 0 on the mesh"));
+        mn = std::fmin(mn, dx[off + i]);
+      }
+      d->dxmin[k] = mn;
+      off += n;
+    } else {
+      double one = 1.0;
+      hipMemcpy(d->dx[k], &one, sizeof(double), hipMemcpyHostToDevice);
+    }
+  }
+  // boundary codes per register
+  std::vector<int> bce((size_t)ne * 6, SAC_BC_CONT), bcb((size_t)d->nbg * 6, SAC_BC_CONT);
+  for (int iw = 0; iw < nw; iw++)
+    for (int iB = 0; iB < 2 * ndim; iB++) {
+      int c = typeB[iw * 2 * ndim + iB];
+      if (iw < ne)
+        bce[(size_t)iw * 6 + iB] = c;
+      else
+        bcb[(size_t)(iw - ne) * 6 + iB] = c;
+    }
+  if (hipMalloc((void**)&d->bc_e, sizeof(int) * bce.size()) != hipSuccess ||
+      hipMalloc((void**)&d->bc_b, sizeof(int) * bcb.size()) != hipSuccess)
+    return bail(fail(SAC_E_NOMEM, "bc alloc"));
+  hipMemcpy(d->bc_e, bce.data(), sizeof(int) * bce.size(), hipMemcpyHostToDevice);
+  hipMemcpy(d->bc_b, bcb.data(), sizeof(int) * bcb.size(), hipMemcpyHostToDevice);
+  if (hipMalloc((void**)&d->red, sizeof(unsigned long long) * (kSlots + 1) * sac::kRedSlots) !=
+      hipSuccess)
+    return bail(fail(SAC_E_NOMEM, "reduction buffer"));
+  hipMemset(d->red, 0, sizeof(unsigned long long) * (kSlots + 1) * sac::kRedSlots);
+  // getdt_grav (src/vacusr.gravity.t:114-121): computed once, at it==itmin
+  d->dtgrav = kBigDouble;
+  for (int k = 0; k < ndim; k++) {
+    double gk = std::fabs(d->grav[k]);
+    if (gk > 0.0) {
+      double mx = -kBigDouble;
+      for (int j = 0; j < ndim; j++) mx = std::fmax(mx, gk / d->dxmin[j]);
+      d->dtgrav = std::fmin(d->dtgrav, 1.0 / std::sqrt(mx));
+    }
+  }
+  if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SAC_E_HIP, "init sync"));
+  *out = d;
+  return 0;
+}
+
+int sac_dev_free(sac_handle h) {
+  if (!h) return 0;
+  Dev* d = (Dev*)h;
+  if (d->stream) hipStreamSynchronize(d->stream);
+  t_collect(*d);
+  for (int s = 0; s < kSlots; s++) hipFree(d->reg[s]);
+  hipFree(d->bg);
+  hipFree(d->ratio);
+  hipFree(d->snap);
+  hipFree(d->bgsnap);
+  for (int k = 0; k < 3; k++) hipFree(d->dx[k]);
+  hipFree(d->bc_e);
+  hipFree(d->bc_b);
+  hipFree(d->red);
+  if (d->stream) hipStreamDestroy(d->stream);
+  delete d;
+  return 0;
+}
+
+int sac_dev_upload(sac_handle h, const double* w) {
+  if (int rc = check_handle(h)) return rc;
+  if (!w) return fail(SAC_E_ARG, "w is null");
+  Dev& d = *(Dev*)h;
+  if (int rc = copy_in(d, d.reg[SAC_SLOT_W], w, d.ne)) return rc;
+  if (int rc = copy_in(d, d.bg, w + (size_t)d.ne * d.ncell_grid, d.nbg)) return rc;
+  d.red_valid[SAC_SLOT_W] = false;
+  return 0;
+}
+
+int sac_dev_download(sac_handle h, double* w) {
+  if (int rc = check_handle(h)) return rc;
+  if (!w) return fail(SAC_E_ARG, "w is null");
+  Dev& d = *(Dev*)h;
+  if (int rc = copy_out(d, w, d.reg[SAC_SLOT_W], d.ne)) return rc;
+  return copy_out(d, w + (size_t)d.ne * d.ncell_grid, d.bg, d.nbg);
+}
+
+int sac_dev_upload_slot(sac_handle h, int slot, const double* w) {
+  if (int rc = check_handle(h)) return rc;
+  if (slot < 0 || slot >= kSlots || !w) return fail(SAC_E_ARG, "bad slot %d", slot);
+  Dev& d = *(Dev*)h;
+  d.red_valid[slot] = false;
+  return copy_in(d, d.reg[slot], w, d.ne);
+}
+
+int sac_dev_download_slot(sac_handle h, int slot, double* w) {
+  if (int rc = check_handle(h)) return rc;
+  if (slot < 0 || slot >= kSlots || !w) return fail(SAC_E_ARG, "bad slot %d", slot);
+  Dev& d = *(Dev*)h;
+  if (int rc = copy_out(d, w, d.reg[slot], d.ne)) return rc;
+  return copy_out(d, w + (size_t)d.ne * d.ncell_grid, d.bg, d.nbg);
+}
+
+int sac_dev_getboundary(sac_handle h) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (!d.initialized) {
+    // first call: background ghosts too, then store 'fixed' values
+    sac::GhostArgs a{};
+    a.g = d.g;
+    a.f = d.bg;
+    a.bc = d.bc_b;
+    a.nvar = d.nbg;
+    a.snap = nullptr;
+    a.keep_fixed = d.fullgridini ? 1 : 0;
+    sac::launch_ghost(a, d.stream);
+    ghost_fill_evolving(d, SAC_SLOT_W);
+    if (d.any_fixed) {
+      HIPCHK(hipMemcpyAsync(d.snap, d.reg[SAC_SLOT_W], sizeof(double) * (size_t)d.ne * d.g.fs,
+                            hipMemcpyDeviceToDevice, d.stream));
+      HIPCHK(hipMemcpyAsync(d.bgsnap, d.bg, sizeof(double) * (size_t)d.nbg * d.g.fs,
+                            hipMemcpyDeviceToDevice, d.stream));
+    }
+    d.initialized = true;
+  } else {
+    ghost_fill_evolving(d, SAC_SLOT_W);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return 0;
+}
+
+int sac_dev_getdt_courant(sac_handle h, double courantpar, double* dt, double* courantmax) {
+  if (int rc = check_handle(h)) return rc;
+  if (!dt) return fail(SAC_E_ARG, "dt is null");
+  Dev& d = *(Dev*)h;
+  if (!d.red_valid[SAC_SLOT_W]) cmax_reduce(d, SAC_SLOT_W);
+  HIPCHK(hipGetLastError());
+  double cm = 0.0;
+  for (int k = 0; k < d.ndim; k++) cm = std::fmax(cm, red_host(d, SAC_SLOT_W, d.ndim + k));
+  double v = kBigDouble;
+  if (cm > kSmallDouble) v = std::fmin(v, courantpar / cm);
+  *dt = v;
+  if (courantmax) *courantmax = cm;
+  return 0;
+}
+
+int sac_dev_getdt_special(sac_handle h, double* dt) {
+  if (int rc = check_handle(h)) return rc;
+  if (!dt) return fail(SAC_E_ARG, "dt is null");
+  Dev& d = *(Dev*)h;
+  double v = *dt;
+  if (d.visc_on) {
+    unsigned long long u = 0;
+    HIPCHK(hipMemcpyAsync(&u, red_visc(d), sizeof(u), hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(hipStreamSynchronize(d.stream));
+    double mv;
+    memcpy(&mv, &u, sizeof(mv));
+    // getdt_visc: dtdiffpar forced to 0.25; +Inf when maxviscoef==0
+    for (int k = 0; k < d.ndim; k++) {
+      double mx = mv / (d.dxmin[k] * d.dxmin[k]);
+      double dtd = 0.25 / mx;
+      v = std::fmin(v, dtd);
+    }
+    sac::launch_zero_u64(red_visc(d), 1, d.stream);
+  }
+  v = std::fmin(v, d.dtgrav);
+  *dt = v;
+  return 0;
+}
+
+int sac_dev_advance(sac_handle h, double dt) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (!d.initialized) return fail(SAC_E_ARG, "getboundary must run before advance");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SAC_E_ARG, "dt=%g", dt);
+  const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, A = SAC_SLOT_ACC;
+  if (d.typeadvance == SAC_ADV_FOURSTEP) {
+    // classical RK4 (src/vac.t:439-455) in two registers plus accumulator
+    stage(d, W, W, S1, dt / 2, W, A, dt / 6);
+    stage(d, S1, W, S2, dt / 2, A, A, dt / 3);
+    stage(d, S2, W, S1, dt, A, A, dt / 3);
+    stage(d, S1, A, W, dt / 6, -1, -1, 0.0);
+  } else {
+    // onestep: w1=w; advect1(dt, wCT=w1, w) (src/vac.t:407-408)
+    HIPCHK(hipMemcpyAsync(d.reg[S1], d.reg[W], sizeof(double) * (size_t)d.ne * d.g.fs,
+                          hipMemcpyDeviceToDevice, d.stream));
+    d.red_valid[S1] = d.red_valid[W];
+    if (d.red_valid[W])
+      HIPCHK(hipMemcpyAsync(red_of(d, S1), red_of(d, W), sizeof(unsigned long long) * sac::kRedSlots,
+                            hipMemcpyDeviceToDevice, d.stream));
+    stage(d, S1, W, W, dt, -1, -1, 0.0);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int sac_dev_advect1(sac_handle h, double qdt, int wct_slot, int w_slot) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (wct_slot < 0 || wct_slot >= kSlots || w_slot < 0 || w_slot >= kSlots || wct_slot == w_slot)
+    return fail(SAC_E_ARG, "advect1 needs two distinct slots (%d,%d)", wct_slot, w_slot);
+  if (!d.initialized) return fail(SAC_E_ARG, "getboundary must run before advect1");
+  stage(d, wct_slot, w_slot, w_slot, qdt, -1, -1, 0.0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return 0;
+}
+
+int sac_dev_get_maxviscoef(sac_handle h, double* v) {
+  if (int rc = check_handle(h)) return rc;
+  if (!v) return fail(SAC_E_ARG, "null");
+  Dev& d = *(Dev*)h;
+  unsigned long long u = 0;
+  HIPCHK(hipMemcpyAsync(&u, red_visc(d), sizeof(u), hipMemcpyDeviceToHost, d.stream));
+  HIPCHK(hipStreamSynchronize(d.stream));
+  memcpy(v, &u, sizeof(double));
+  return 0;
+}
+
+int sac_dev_set_maxviscoef(sac_handle h, double v) {
+  if (int rc = check_handle(h)) return rc;
+  if (!(v >= 0.0)) return fail(SAC_E_ARG, "maxviscoef must be >= 0");
+  Dev& d = *(Dev*)h;
+  unsigned long long u;
+  memcpy(&u, &v, sizeof(u));
+  HIPCHK(hipMemcpyAsync(red_visc(d), &u, sizeof(u), hipMemcpyHostToDevice, d.stream));
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return 0;
+}
+
+int sac_dev_timing(sac_handle h, int enable) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  hipStreamSynchronize(d.stream);
+  t_collect(d);
+  for (int k = 0; k < 4; k++) {
+    d.t_ms[k] = 0.0;
+    d.t_n[k] = 0;
+  }
+  d.timing = enable != 0;
+  return 0;
+}
+
+int sac_dev_timing_get(sac_handle h, int kernel, double* total_ms, long long* count) {
+  if (int rc = check_handle(h)) return rc;
+  if (kernel < 0 || kernel > 3) return fail(SAC_E_ARG, "kernel id %d", kernel);
+  Dev& d = *(Dev*)h;
+  t_collect(d);
+  if (total_ms) *total_ms = d.t_ms[kernel];
+  if (count) *count = d.t_n[kernel];
+  return 0;
+}
+
+int sac_dev_synchronize(sac_handle h) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  HIPCHK(hipStreamSynchronize(d.stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int sac_dev_memory(sac_handle h, long long* bytes) {
+  if (int rc = check_handle(h)) return rc;
+  if (bytes) *bytes = ((Dev*)h)->bytes;
+  return 0;
+}
+
+}  // extern "C"
