@@ -16,7 +16,7 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libsac_oracle.so")
+LIB_PATH = os.environ.get("SAC_ORACLE_LIB") or os.path.join(_HERE, "_build", "libsac_oracle.so")
 
 BC = {"cont": 1, "fixed": 2, "periodic": 3, "symm": 4, "symm0": 5, "asymm": 6}
 ADV = {"onestep": 1, "twostep": 2, "threestep": 3, "fourstep": 4,
@@ -29,6 +29,8 @@ def build(force: bool = False) -> str:
     """Compile the oracle with gcc into oracle/_build (plain C, -O2, no
     fast-math, no FMA contraction so that the Fortran evaluation order is
     kept)."""
+    if os.environ.get("SAC_ORACLE_LIB"):
+        return LIB_PATH  # a prebuilt variant chosen by the caller
     os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
     src = os.path.join(_HERE, "sac_oracle.c")
     if force or not os.path.exists(LIB_PATH) or \
