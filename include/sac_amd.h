@@ -52,10 +52,11 @@ extern "C" {
 #define SAC_SLOT_ACC 3
 
 /* kernels reported by sac_dev_timing_get */
-#define SAC_K_STAGE 0
-#define SAC_K_RATIO 1
+#define SAC_K_FINAL 0 /* cross terms, gravity, register update, cmax reductions */
+#define SAC_K_AXIS 1  /* per-axis fluxes, setnu and hyperdiffusion (ndim launches) */
 #define SAC_K_GHOST 2
 #define SAC_K_CMAX 3
+#define SAC_K_COPY 4
 
 typedef void *sac_handle;
 
@@ -122,6 +123,12 @@ int sac_dev_advance(sac_handle h, double dt);
 /* advect1 with method 'cd4' (src/vac.t:490-549) on device registers:
  * slot w <- w + qdt*R(slot wCT), then getboundary(w). */
 int sac_dev_advect1(sac_handle h, double qdt, int wct_slot, int w_slot);
+
+/* Register copy of the evolving variables, whole grid: the reference's
+ * w1(ixG^S,1:nw)=w(ixG^S,1:nw) (src/vac.t:403,444,447).  Done by a plain
+ * streaming kernel, which also serves as the known-size copy that HBM
+ * counters are calibrated against. */
+int sac_dev_copy_slot(sac_handle h, int dst_slot, int src_slot);
 
 /* maxviscoef (src/vacdef.t:176) accumulated by setnu since the last
  * getdt_special. */

@@ -63,7 +63,8 @@ struct Dev {
   // device buffers
   double* reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   double* bg = nullptr;
-  double* ratio = nullptr;
+  double* rate = nullptr;   // R accumulator
+  double* cross = nullptr;  // cross-term factors T
   double* snap = nullptr;   // 'fixed' values of the evolving fields
   double* bgsnap = nullptr;
   double* dx[3] = {nullptr, nullptr, nullptr};
@@ -75,8 +76,8 @@ struct Dev {
   long long bytes = 0;
   bool timing = false;
   std::vector<TimedLaunch> timed;
-  double t_ms[4] = {0, 0, 0, 0};
-  long long t_n[4] = {0, 0, 0, 0};
+  double t_ms[5] = {0, 0, 0, 0, 0};
+  long long t_n[5] = {0, 0, 0, 0, 0};
 };
 
 long long fieldsz(const Dev& d) { return d.g.fs; }
@@ -154,28 +155,35 @@ void cmax_reduce(Dev& d, int slot) {
 // one stage: out = base + qa*R(wct) ; acc_out = acc_in + qb*R ; ghosts of out
 void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_out, double qb) {
   if (!d.red_valid[wct]) cmax_reduce(d, wct);
-  if (d.visc_on) {
-    sac::RatioArgs r{};
-    r.g = d.g;
-    r.wct = d.reg[wct];
-    r.bg = d.bg;
-    r.ratio = d.ratio;
+  sac::AxisArgs x{};
+  x.g = d.g;
+  x.wct = d.reg[wct];
+  x.bg = d.bg;
+  for (int k = 0; k < 3; k++) x.dx[k] = d.dx[k];
+  x.R = d.rate;
+  x.T = d.cross;
+  x.gamma = d.gamma;
+  x.visc_on = d.visc_on;
+  x.ctot_src = red_of(d, wct);
+  x.red_visc = red_visc(d);
+  for (int k = 0; k < d.ndim; k++) {
     hipEvent_t e0{};
-    t_begin(d, SAC_K_RATIO, &e0);
-    sac::launch_ratio(r, d.stream);
-    t_end(d, SAC_K_RATIO, e0);
+    t_begin(d, SAC_K_AXIS, &e0);
+    sac::launch_axis(x, k, d.stream);
+    t_end(d, SAC_K_AXIS, e0);
   }
   sac::launch_zero_u64(red_of(d, out), sac::kRedSlots, d.stream);
-  sac::StageArgs s{};
+  sac::FinalArgs s{};
   s.g = d.g;
   s.wct = d.reg[wct];
   s.bg = d.bg;
+  for (int k = 0; k < 3; k++) s.dx[k] = d.dx[k];
+  s.R = d.rate;
+  s.T = d.cross;
   s.base = d.reg[base];
   s.out = d.reg[out];
   s.acc_in = acc_in >= 0 ? d.reg[acc_in] : nullptr;
   s.acc_out = acc_out >= 0 ? d.reg[acc_out] : nullptr;
-  s.ratio = d.ratio;
-  for (int k = 0; k < 3; k++) s.dx[k] = d.dx[k];
   s.qa = qa;
   s.qb = qb;
   s.gamma = d.gamma;
@@ -184,14 +192,11 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
     s.grav_on[k] = d.grav_on[k];
   }
   s.visc_on = d.visc_on;
-  s.ctot_src = red_of(d, wct);
   s.red_out = red_of(d, out);
-  s.red_visc = red_visc(d);
-  s.reduce_out = 1;
   hipEvent_t e0{};
-  t_begin(d, SAC_K_STAGE, &e0);
-  sac::launch_stage(s, d.stream);
-  t_end(d, SAC_K_STAGE, e0);
+  t_begin(d, SAC_K_FINAL, &e0);
+  sac::launch_final(s, d.stream);
+  t_end(d, SAC_K_FINAL, e0);
   d.red_valid[out] = true;
   if (acc_out >= 0) d.red_valid[acc_out] = false;
   ghost_fill_evolving(d, out);
@@ -351,7 +356,10 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   for (int s = 0; s < kSlots; s++)
     if ((rc = dalloc(*d, &d->reg[s], (long long)ne * g.fs))) return bail(rc);
   if ((rc = dalloc(*d, &d->bg, (long long)d->nbg * g.fs))) return bail(rc);
-  if (d->visc_on && (rc = dalloc(*d, &d->ratio, (long long)d->npairs * g.fs))) return bail(rc);
+  if ((rc = dalloc(*d, &d->rate, (long long)ne * g.fs))) return bail(rc);
+  if (d->visc_on && ndim > 1 &&
+      (rc = dalloc(*d, &d->cross, (long long)2 * ndim * (ndim - 1) * g.fs)))
+    return bail(rc);
   if (d->any_fixed) {
     if ((rc = dalloc(*d, &d->snap, (long long)ne * g.fs))) return bail(rc);
     if ((rc = dalloc(*d, &d->bgsnap, (long long)d->nbg * g.fs))) return bail(rc);
@@ -417,7 +425,8 @@ int sac_dev_free(sac_handle h) {
   t_collect(*d);
   for (int s = 0; s < kSlots; s++) hipFree(d->reg[s]);
   hipFree(d->bg);
-  hipFree(d->ratio);
+  hipFree(d->rate);
+  hipFree(d->cross);
   hipFree(d->snap);
   hipFree(d->bgsnap);
   for (int k = 0; k < 3; k++) hipFree(d->dx[k]);
@@ -569,6 +578,22 @@ int sac_dev_advect1(sac_handle h, double qdt, int wct_slot, int w_slot) {
   return 0;
 }
 
+int sac_dev_copy_slot(sac_handle h, int dst_slot, int src_slot) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (dst_slot < 0 || dst_slot >= kSlots || src_slot < 0 || src_slot >= kSlots)
+    return fail(SAC_E_ARG, "bad slots %d %d", dst_slot, src_slot);
+  if (dst_slot == src_slot) return 0;
+  hipEvent_t e0{};
+  t_begin(d, SAC_K_COPY, &e0);
+  sac::launch_axpy4((long long)d.ne * d.g.fs, d.reg[dst_slot], d.reg[src_slot], 1.0, nullptr, 0.0,
+                    nullptr, 0.0, nullptr, 0.0, d.stream);
+  t_end(d, SAC_K_COPY, e0);
+  d.red_valid[dst_slot] = false;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int sac_dev_get_maxviscoef(sac_handle h, double* v) {
   if (int rc = check_handle(h)) return rc;
   if (!v) return fail(SAC_E_ARG, "null");
@@ -596,7 +621,7 @@ int sac_dev_timing(sac_handle h, int enable) {
   Dev& d = *(Dev*)h;
   hipStreamSynchronize(d.stream);
   t_collect(d);
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < 5; k++) {
     d.t_ms[k] = 0.0;
     d.t_n[k] = 0;
   }
@@ -606,7 +631,7 @@ int sac_dev_timing(sac_handle h, int enable) {
 
 int sac_dev_timing_get(sac_handle h, int kernel, double* total_ms, long long* count) {
   if (int rc = check_handle(h)) return rc;
-  if (kernel < 0 || kernel > 3) return fail(SAC_E_ARG, "kernel id %d", kernel);
+  if (kernel < 0 || kernel > 4) return fail(SAC_E_ARG, "kernel id %d", kernel);
   Dev& d = *(Dev*)h;
   t_collect(d);
   if (total_ms) *total_ms = d.t_ms[kernel];

@@ -4,32 +4,36 @@
 
 namespace sac {
 
-struct RatioArgs {
-  GridDesc g;
-  const double* wct;  // evolving fields of the stage input
-  const double* bg;   // background fields
-  double* ratio;      // ndim*(2*ndim+1) face-ratio fields
-};
-
-struct StageArgs {
+struct AxisArgs {
   GridDesc g;
   const double* wct;     // stage input (time-centred state)
+  const double* bg;      // background fields
+  const double* dx[3];   // per-axis cell widths, indexed by grid index
+  double* R;             // rate accumulator (n_evolve fields)
+  double* T;             // cross-term factors (2*ndim*(ndim-1) fields)
+  double gamma;
+  int visc_on;
+  const unsigned long long* ctot_src;  // c_tot per direction of wct
+  unsigned long long* red_visc;        // maxviscoef accumulator
+};
+
+struct FinalArgs {
+  GridDesc g;
+  const double* wct;
   const double* bg;
+  const double* dx[3];
+  const double* R;
+  const double* T;
   const double* base;    // out = base + qa*R
   double* out;
   const double* acc_in;  // acc_out = acc_in + qb*R (acc_out may be null)
   double* acc_out;
-  const double* ratio;
-  const double* dx[3];   // per-axis cell widths, indexed by grid index
   double qa, qb;
   double gamma;
   double grav[3];
   int grav_on[3];
   int visc_on;
-  const unsigned long long* ctot_src;  // c_tot per direction of wct
-  unsigned long long* red_out;         // cmax reductions of out
-  unsigned long long* red_visc;        // maxviscoef accumulator
-  int reduce_out;
+  unsigned long long* red_out;  // cmax reductions of out
 };
 
 struct CmaxArgs {
@@ -50,8 +54,8 @@ struct GhostArgs {
   int keep_fixed;      // initial fill with fullgridini: keep file values
 };
 
-void launch_ratio(const RatioArgs& a, hipStream_t st);
-void launch_stage(const StageArgs& a, hipStream_t st);
+void launch_axis(const AxisArgs& a, int axis, hipStream_t st);
+void launch_final(const FinalArgs& a, hipStream_t st);
 void launch_cmax(const CmaxArgs& a, hipStream_t st);
 void launch_ghost(const GhostArgs& a, hipStream_t st);
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);

@@ -17,6 +17,14 @@
 //   getboundary        src/vacgrid.t:195-512
 #include "sac_kernels.h"
 
+// keeps the machine scheduler from hoisting the next phase's loads above
+// the current phase's arithmetic, which bounds the number of live values
+#define SCHED_FENCE()                  \
+  do {                                 \
+    asm volatile("" ::: "memory");     \
+    __builtin_amdgcn_sched_barrier(0); \
+  } while (0)
+
 namespace sac {
 
 // ---------------------------------------------------------------------
@@ -164,108 +172,80 @@ __device__ __forceinline__ long long lin(const GridDesc& g, int i0, int i1, int 
 }
 
 // ---------------------------------------------------------------------
-// kernel 1: setnu face ratios  r = md3R/md1R at faces j+1/2 along k
-// (src/vacusr.viscosity.t:296-427); cells j in [Mlo-1, Mhi] along k and
-// the mesh along the other axes.  q beyond the grid is the mirror value of
-// the cell six inside (src/vacusr.viscosity.t:378-400).
+// setnu variables at one point (q of setnu for every variable)
+
+template <int NDIR>
+struct Qv {
+  double rho, qe, rt, inv, v[3], m[3], b[3];
+};
 
 template <int NDIM, int NDIR>
-__global__ void __launch_bounds__(256) k_ratio(RatioArgs a) {
-  const GridDesc& g = a.g;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int lo[3], ext[3];
+__device__ __forceinline__ void load_qv(const double* __restrict__ W, const double* __restrict__ B,
+                                        long long fs, long long i, Qv<NDIR>& q) {
+  q.rho = W[i];
+  q.rt = q.rho + B[fs + i];
+  q.inv = 1.0 / q.rt;
+  double sb = 0.0, sm = 0.0;
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    lo[d] = (d < NDIM) ? g.Mlo[d] - 1 : 0;
-    ext[d] = (d < NDIM) ? g.Mhi[d] - g.Mlo[d] + 2 : 1;
-  }
-  if (t >= (long long)ext[0] * ext[1] * ext[2]) return;
-  int c[3];
-  decode_box(t, lo, ext, c);
-  const long long fs = g.fs;
-  const double* __restrict__ W = a.wct;
-  const double* __restrict__ B = a.bg;
-  constexpr int NP = 2 * NDIM + 1;
-#pragma unroll
-  for (int k = 0; k < NDIM; k++) {
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < NDIM; d++)
-      if (d != k && c[d] < g.Mlo[d]) ok = false;
-    if (!ok) continue;
-    // q values at j-2..j+3 along k
-    double q[NP][6];
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
-      int cc[3] = {c[0], c[1], c[2]};
-      int jj = c[k] + s - 2;
-      if (jj < 0) jj = g.Mlo[k] + 3;             // mirror below the grid
-      if (jj > g.G[k] - 1) jj = g.Mhi[k] - 3;    // mirror above the grid
-      cc[k] = jj;
-      const long long i = lin(g, cc[0], cc[1], cc[2]);
-      const double rho = W[i];
-      const double rt = rho + B[fs + i];
-      const double inv = 1.0 / rt;
-      double m[NDIR], b[NDIR];
-      double sb = 0.0, sm = 0.0;
-#pragma unroll
-      for (int cdir = 0; cdir < NDIR; cdir++) {
-        m[cdir] = W[(1 + cdir) * fs + i];
-        b[cdir] = W[(2 + NDIR + cdir) * fs + i];
-        sb += b[cdir] * b[cdir];
-        sm += m[cdir] * m[cdir];
-      }
-      const double e = W[(1 + NDIR) * fs + i];
-      q[0][s] = rho;
-      q[1][s] = e - 0.5 * (sb + sm * inv);
-#pragma unroll
-      for (int l = 0; l < NDIM; l++) q[2 + l][s] = m[l] * inv;
-      int n = 2 + NDIM;
-#pragma unroll
-      for (int l = 0; l < NDIM; l++)
-        if (l != k) q[n++][s] = b[l];
-    }
-    const long long io = lin(g, c[0], c[1], c[2]);
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-      double d1[5], d3[3];
-#pragma unroll
-      for (int j = 0; j < 5; j++) d1[j] = fabs(q[p][j + 1] - q[p][j]);
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-        d3[j] = fabs(3.0 * (q[p][j + 2] - q[p][j + 1]) - (q[p][j + 3] - q[p][j]));
-      double md3 = fmax(fmax(d3[0], d3[1]), d3[2]);
-      double md1 = fmax(fmax(fmax(d1[0], d1[1]), fmax(d1[2], d1[3])), d1[4]);
-      const double r = (md1 > 0.0) ? md3 / md1 : 0.0;
-      a.ratio[(long long)(k * NP + p) * fs + io] = r;
+  for (int c = 0; c < NDIR; c++) {
+    const double mc = W[(1 + c) * fs + i];
+    const double bc = W[(2 + NDIR + c) * fs + i];
+    sb += bc * bc;
+    sm += mc * mc;
+    if (c < NDIM) {
+      q.m[c] = mc;
+      q.v[c] = mc * q.inv;
+      q.b[c] = bc;
     }
   }
+  q.qe = W[(1 + NDIR) * fs + i] - 0.5 * (sb + sm * q.inv);
+}
+
+// setnu ratio md3/md1 at the face between q[j] and q[j+1] of a 6-value
+// window q[j-2..j+3] (src/vacusr.viscosity.t:413-427)
+__device__ __forceinline__ double face_ratio(const double* q) {
+  double d1[5], d3[3];
+#pragma unroll
+  for (int j = 0; j < 5; j++) d1[j] = fabs(q[j + 1] - q[j]);
+#pragma unroll
+  for (int j = 0; j < 3; j++) d3[j] = fabs(3.0 * (q[j + 2] - q[j + 1]) - (q[j + 3] - q[j]));
+  const double md3 = fmax(fmax(d3[0], d3[1]), d3[2]);
+  const double md1 = fmax(fmax(fmax(d1[0], d1[1]), fmax(d1[2], d1[3])), d1[4]);
+  return (md1 > 0.0) ? md3 / md1 : 0.0;
+}
+
+__device__ __forceinline__ int cross_index(int ndim, int k, int l) {
+  return 2 * (k * (ndim - 1) + (l < k ? l : l - 1));
 }
 
 // ---------------------------------------------------------------------
-// kernel 2: fused stage update on the mesh.
-//   R = -sum_d D4_d(F_d) + background energy terms + gravity + hyperdiffusion
-//   out = base + a*R ; acc_out = acc_in + b*R (optional)
-// plus max reductions of cmax_d(out) and cmax_d(out)/dx_d, and of every nu.
+// kernel 1: the work of one axis K for every mesh cell, all of which lies
+// on the K-line through the cell:
+//   - 4th-order flux divergence along K (centdiff4, src/vaccd.t:35-70)
+//     and the background energy terms with derivatives along K,
+//   - setnu (src/vacusr.viscosity.t:224-455) for the 2*ndim+1 variables of
+//     axis K from the 7-point stencil (mirror layer beyond the grid),
+//   - the hyperdiffusion terms along K of addsource_visc (rho, thermal
+//     energy, momentum i==k branch, field m==k branch),
+//   - the cross-term factors T (diffusion along K) that kernel 2
+//     differentiates along the other axes:
+//       Tm_{K,l} = d_K v_l * (nuL+nuR)_{m_l}/4,  Tb_{K,l} = d_K b_l * (nuL+nuR)_{b_l}/2.
+// The contribution is added to the rate accumulator R (written on K=0).
 
-template <int NDIM, int NDIR>
-__global__ void __launch_bounds__(256) k_stage(StageArgs a) {
+template <int NDIM, int NDIR, int K>
+__global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
   constexpr int NE = 2 + 2 * NDIR;
   constexpr int NP = 2 * NDIM + 1;
   const GridDesc& g = a.g;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int lo[3], ext[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     lo[d] = g.Mlo[d];
     ext[d] = g.Mhi[d] - g.Mlo[d] + 1;
   }
-  const long long ncell = (long long)ext[0] * ext[1] * ext[2];
-  const bool active = t < ncell;
-  double redv[2 * NDIM + 1];
-#pragma unroll
-  for (int k = 0; k < 2 * NDIM + 1; k++) redv[k] = 0.0;
-
+  const bool active = t < (long long)ext[0] * ext[1] * ext[2];
+  double numax = 0.0;
   if (active) {
     int c[3];
     decode_box(t, lo, ext, c);
@@ -274,241 +254,257 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs a) {
     const double* __restrict__ B = a.bg;
     const double gamma = a.gamma;
     const long long ic = lin(g, c[0], c[1], c[2]);
+    const long long sk = g.st[K];
+    const double dxk = a.dx[K][c[K]];
+    const double idx = 1.0 / dxk;
     double R[NE];
 #pragma unroll
     for (int v = 0; v < NE; v++) R[v] = 0.0;
 
-    Pt<NDIR> p0;
-    load_pt<NDIR>(W, B, fs, ic, p0);
-    const double rt0 = p0.rho + p0.rhob;
-    const double inv0 = 1.0 / rt0;
-    const double pTb0 = ptotal_bg<NDIR>(p0, gamma);
-    double dxc[3];
+    // ---- flux divergence along K (zero on two layers at each side) ----
+    if (!((c[K] <= g.Mlo[K] + 1) || (c[K] >= g.Mhi[K] - 1))) {
+      double bgk, bgl[NDIM], pTb0;
+      {
+        Pt<NDIR> p0;
+        load_pt<NDIR>(W, B, fs, ic, p0);
+        pTb0 = ptotal_bg<NDIR>(p0, gamma);
+        bgk = p0.bg[K];
 #pragma unroll
-    for (int d = 0; d < NDIM; d++) dxc[d] = a.dx[d][c[d]];
-
-    // ---- flux divergence and background energy terms (centdiff4) ----
-    // per axis k: points at c-2e_k, c-e_k, c+e_k, c+2e_k; the +-1
-    // neighbours also supply the hyperdiffusion variables
-    double nb_rho[NDIM][2], nb_qe[NDIM][2], nb_v[NDIM][NDIM][2], nb_b[NDIM][NDIM][2],
-        nb_rt[NDIM][2], nb_m[NDIM][NDIM][2];
-#pragma unroll
-    for (int k = 0; k < NDIM; k++) {
-      const bool edge4 = (c[k] <= g.Mlo[k] + 1) || (c[k] >= g.Mhi[k] - 1);
-      double Fs[4][NE];
-      double vs[4][NDIM];
-#pragma unroll
-      for (int s = 0; s < 4; s++) {
-        const int off = (s < 2) ? s - 2 : s - 1;  // -2,-1,+1,+2
-        if (edge4 && (off == -2 || off == 2)) continue;
-        const long long i = ic + (long long)off * g.st[k];
-        Pt<NDIR> p;
-        load_pt<NDIR>(W, B, fs, i, p);
-        double F[NE];
-        flux_pt<NDIR>(p, k, gamma, F);
-#pragma unroll
-        for (int v = 0; v < NE; v++) Fs[s][v] = F[v];
-        const double rt = p.rho + p.rhob;
-        const double inv = 1.0 / rt;
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) vs[s][l] = p.m[l] * inv;
-        if (off == -1 || off == 1) {
-          const int sd = (off == -1) ? 0 : 1;
-          double sb = 0.0, sm = 0.0;
-#pragma unroll
-          for (int cdir = 0; cdir < NDIR; cdir++) {
-            sb += p.b[cdir] * p.b[cdir];
-            sm += p.m[cdir] * p.m[cdir];
-          }
-          nb_rho[k][sd] = p.rho;
-          nb_qe[k][sd] = p.e - 0.5 * (sb + sm * inv);
-          nb_rt[k][sd] = rt;
-#pragma unroll
-          for (int l = 0; l < NDIM; l++) {
-            nb_v[k][l][sd] = p.m[l] * inv;
-            nb_m[k][l][sd] = p.m[l];
-            nb_b[k][l][sd] = p.b[l];
-          }
-        }
+        for (int l = 0; l < NDIM; l++) bgl[l] = p0.bg[l];
       }
-      if (!edge4) {
-        const double inv12 = 1.0 / dxc[k] / 12.0;
+      const double inv12 = idx / 12.0;
+      double Fm[NE], D1[NE], vm[NDIM], dv1[NDIM];
+      SCHED_FENCE();
+      {
+        Pt<NDIR> p;
+        load_pt<NDIR>(W, B, fs, ic - sk, p);
+        flux_pt<NDIR>(p, K, gamma, Fm);
+        const double inv = 1.0 / (p.rho + p.rhob);
 #pragma unroll
-        for (int v = 0; v < NE; v++) {
-          const double D = -(Fs[3][v] - 8.0 * (Fs[2][v] - Fs[1][v]) - Fs[0][v]) * inv12;
-          R[v] -= D;
+        for (int l = 0; l < NDIM; l++) vm[l] = p.m[l] * inv;
+      }
+      SCHED_FENCE();
+      {
+        Pt<NDIR> p;
+        double F[NE];
+        load_pt<NDIR>(W, B, fs, ic + sk, p);
+        flux_pt<NDIR>(p, K, gamma, F);
+        const double inv = 1.0 / (p.rho + p.rhob);
+#pragma unroll
+        for (int v = 0; v < NE; v++) D1[v] = F[v] - Fm[v];
+#pragma unroll
+        for (int l = 0; l < NDIM; l++) dv1[l] = p.m[l] * inv - vm[l];
+      }
+      SCHED_FENCE();
+      {
+        Pt<NDIR> p;
+        load_pt<NDIR>(W, B, fs, ic - 2 * sk, p);
+        flux_pt<NDIR>(p, K, gamma, Fm);
+        const double inv = 1.0 / (p.rho + p.rhob);
+#pragma unroll
+        for (int l = 0; l < NDIM; l++) vm[l] = p.m[l] * inv;
+      }
+      SCHED_FENCE();
+      {
+        Pt<NDIR> p;
+        double F[NE];
+        load_pt<NDIR>(W, B, fs, ic + 2 * sk, p);
+        flux_pt<NDIR>(p, K, gamma, F);
+        const double inv = 1.0 / (p.rho + p.rhob);
+        // -gradient4 = ((F2-F-2) - 8 (F1-F-1)) / (12 dx)
+#pragma unroll
+        for (int v = 0; v < NE; v++) R[v] += ((F[v] - Fm[v]) - 8.0 * D1[v]) * inv12;
+        double re = 0.0;
+#pragma unroll
+        for (int l = 0; l < NDIM; l++) {
+          const double Dv = -((p.m[l] * inv - vm[l]) - 8.0 * dv1[l]) * inv12;
+          if (l == K) re -= Dv * pTb0;
+          re += bgk * bgl[l] * Dv;
         }
-        // background energy terms with derivative along k
-        double Dv[NDIM];
-#pragma unroll
-        for (int l = 0; l < NDIM; l++)
-          Dv[l] = -(vs[3][l] - 8.0 * (vs[2][l] - vs[1][l]) - vs[0][l]) * inv12;
-        double re = -Dv[k] * pTb0;
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) re += p0.bg[k] * p0.bg[l] * Dv[l];
         R[1 + NDIR] += re;
       }
     }
+    SCHED_FENCE();
 
-    // ---- gravity on the perturbed density (addsource_grav) ----
+    // ---- hyperdiffusion along K ----
+    if (a.visc_on) {
+      const double ctk = red_get(a.ctot_src, K);
+      // rho_t and 1/rho_t at offsets -3..3 (mirror beyond the grid)
+      long long id7[7];
+      double inv7[7];
+#pragma unroll
+      for (int s = 0; s < 7; s++) {
+        int j = c[K] + s - 3;
+        if (j < 0) j = g.Mlo[K] + 3;
+        if (j > g.G[K] - 1) j = g.Mhi[K] - 3;
+        id7[s] = ic + (long long)(j - c[K]) * sk;
+        inv7[s] = 1.0 / (W[id7[s]] + B[fs + id7[s]]);
+      }
+      const bool z1 = (c[K] == g.Mlo[K] && !g.per_lo[K]) || (c[K] == g.Mhi[K] && !g.per_hi[K]);
+      const double rtm = W[ic - sk] + B[fs + ic - sk];
+      const double rt0 = W[ic] + B[fs + ic];
+      const double rtp = W[ic + sk] + B[fs + ic + sk];
+      // p = 0: rho' ; p = 1: thermal energy ; p = 2+l: v_l ; then b_l (l != K)
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        double q[7];
+        int lb = -1;
+        if (p >= 2 + NDIM) {
+          int n = p - 2 - NDIM;
+          lb = (n < K) ? n : n + 1;
+        }
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+          const long long i = id7[s];
+          if (p == 0) {
+            q[s] = W[i];
+          } else if (p == 1) {
+            double sb = 0.0, sm = 0.0;
+#pragma unroll
+            for (int cd = 0; cd < NDIR; cd++) {
+              const double mc = W[(1 + cd) * fs + i];
+              const double bc = W[(2 + NDIR + cd) * fs + i];
+              sb += bc * bc;
+              sm += mc * mc;
+            }
+            q[s] = W[(1 + NDIR) * fs + i] - 0.5 * (sb + sm * inv7[s]);
+          } else if (p < 2 + NDIM) {
+            q[s] = W[(p - 1) * fs + i] * inv7[s];
+          } else {
+            q[s] = W[(2 + NDIR + lb) * fs + i];
+          }
+        }
+        const double ch = (p >= 2 && p < 2 + NDIM) ? 0.4 : 0.04;
+        const double Kc = ctk * ch;
+        const double nuR = Kc * face_ratio(q + 1) * dxk;
+        const double nuL = Kc * face_ratio(q) * dxk;
+        numax = fmax(numax, fmax(nuR, nuL));
+        const double gL = z1 ? 0.0 : (q[3] - q[2]) * idx;
+        const double gR = z1 ? 0.0 : (q[4] - q[3]) * idx;
+        const double tL = nuL * gL, tR = nuR * gR;
+        if (p == 0) {
+          R[0] += (tR - tL) * idx;
+        } else if (p == 1) {
+          R[1 + NDIR] += (tR - tL) * idx;
+        } else if (p < 2 + NDIM) {
+          const int l = p - 2;
+          const double rhoL = (rt0 + rtm) * 0.5, rhoR = (rtp + rt0) * 0.5;
+          const double m0 = W[(1 + l) * fs + ic];
+          const double mL = (m0 + W[(1 + l) * fs + ic - sk]) * 0.5;
+          const double mR = (W[(1 + l) * fs + ic + sk] + m0) * 0.5;
+          const double mult = (l == K) ? 1.0 : 0.5;  // the i==k branch runs twice for l==k
+          R[1 + l] += mult * (rhoR * tR - rhoL * tL) * idx;
+          R[1 + NDIR] += mult * (mR * tR - mL * tL) * idx;
+          if (l != K) {
+            const double gC = z1 ? 0.0 : (q[4] - q[2]) * idx * 0.5;
+            a.T[(long long)cross_index(NDIM, K, l) * fs + ic] = gC * (nuL + nuR) * 0.25;
+          }
+        } else {
+          const double BL = (q[3] + q[2]) * 0.5, BR = (q[4] + q[3]) * 0.5;
+          R[2 + NDIR + lb] += (tR - tL) * idx;
+          R[1 + NDIR] += (tR * BR - tL * BL) * idx;
+          const double gC = z1 ? 0.0 : (q[4] - q[2]) * idx * 0.5;
+          a.T[(long long)(cross_index(NDIM, K, lb) + 1) * fs + ic] = gC * (nuL + nuR) * 0.5;
+        }
+        SCHED_FENCE();
+      }
+    }
+    // ---- accumulate ----
+    double* __restrict__ Racc = a.R;
+    if (K == 0) {
+#pragma unroll
+      for (int v = 0; v < NE; v++) Racc[v * fs + ic] = R[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NE; v++) Racc[v * fs + ic] += R[v];
+    }
+  }
+  if (a.visc_on) {
+    double r2[1] = {numax};
+    block_max_atomic<1>(r2, a.red_visc);
+  }
+}
+
+// ---------------------------------------------------------------------
+// kernel 2: gravity, derivatives of the cross-term factors, the register
+// update out = base + qa*R (acc_out = acc_in + qb*R) and the cmax
+// reductions of the result.
+//   m_k += d_l(rho_t Tm_{k,l}),  e += d_l(m_k Tm_{k,l})
+//   b_k -= d_l(Tb_{k,l}),        e -= d_l(b_k Tb_{k,l})
+// with central differences along l, zero at a non-periodic edge of l, and
+// T = 0 outside the mesh (nu is never set there, src/vacusr.viscosity.t:116-129).
+
+template <int NDIM, int NDIR>
+__global__ void __launch_bounds__(256) k_final(FinalArgs a) {
+  constexpr int NE = 2 + 2 * NDIR;
+  const GridDesc& g = a.g;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int lo[3], ext[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    lo[d] = g.Mlo[d];
+    ext[d] = g.Mhi[d] - g.Mlo[d] + 1;
+  }
+  const bool active = t < (long long)ext[0] * ext[1] * ext[2];
+  double redv[2 * NDIM];
+#pragma unroll
+  for (int k = 0; k < 2 * NDIM; k++) redv[k] = 0.0;
+  if (active) {
+    int c[3];
+    decode_box(t, lo, ext, c);
+    const long long fs = g.fs;
+    const double* __restrict__ W = a.wct;
+    const double* __restrict__ B = a.bg;
+    const long long ic = lin(g, c[0], c[1], c[2]);
+    double R[NE];
+#pragma unroll
+    for (int v = 0; v < NE; v++) R[v] = a.R[v * fs + ic];
+    // gravity on the perturbed density (addsource_grav, src/vacusr.gravity.t:46-78)
     {
+      const double rho = W[ic];
+      const double inv = 1.0 / (rho + B[fs + ic]);
       double rg = 0.0;
 #pragma unroll
       for (int d = 0; d < NDIM; d++) {
         if (a.grav_on[d]) {
-          R[1 + d] += a.grav[d] * p0.rho;
-          rg += a.grav[d] * p0.rho * p0.m[d] * inv0;
+          R[1 + d] += a.grav[d] * rho;
+          rg += a.grav[d] * rho * W[(1 + d) * fs + ic] * inv;
         }
       }
       R[1 + NDIR] += rg;
     }
-
-    // ---- hyperdiffusion (addsource_visc) ----
-    double numax = 0.0;
-    if (a.visc_on) {
-      double qe0;
-      {
-        double sb = 0.0, sm = 0.0;
+    if (a.visc_on && NDIM > 1) {
 #pragma unroll
-        for (int cdir = 0; cdir < NDIR; cdir++) {
-          sb += p0.b[cdir] * p0.b[cdir];
-          sm += p0.m[cdir] * p0.m[cdir];
-        }
-        qe0 = p0.e - 0.5 * (sb + sm * inv0);
-      }
-      double ctot[NDIM];
+      for (int l = 0; l < NDIM; l++) {
+        if ((c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l])) continue;
+        const long long sl = g.st[l];
+        const double h = 0.5 / a.dx[l][c[l]];
+        const bool inP = c[l] + 1 <= g.Mhi[l], inM = c[l] - 1 >= g.Mlo[l];
+        const long long ip = ic + sl, im = ic - sl;
+        const double rtp = W[ip] + B[fs + ip], rtm = W[im] + B[fs + im];
 #pragma unroll
-      for (int k = 0; k < NDIM; k++) ctot[k] = red_get(a.ctot_src, k);
-      const double* __restrict__ RT = a.ratio;
-
-#pragma unroll
-      for (int k = 0; k < NDIM; k++) {
-        // gradient1L/1R are zero at the first/last mesh cell of a
-        // non-periodic side
-        const bool z1 = (c[k] == g.Mlo[k] && !g.per_lo[k]) || (c[k] == g.Mhi[k] && !g.per_hi[k]);
-        const long long im = ic - g.st[k];
-        const double dxk = dxc[k];
-        double nuR[NP], nuL[NP];
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-          const double ch = (p >= 2 && p < 2 + NDIM) ? 0.4 : 0.04;
-          const double K = ctot[k] * ch;
-          nuR[p] = K * RT[(long long)(k * NP + p) * fs + ic] * dxk;
-          nuL[p] = K * RT[(long long)(k * NP + p) * fs + im] * dxk;
-          numax = fmax(numax, fmax(nuR[p], nuL[p]));
-        }
-        if (!z1) {
-          // rho
-          {
-            const double gL = (p0.rho - nb_rho[k][0]) / dxk;
-            const double gR = (nb_rho[k][1] - p0.rho) / dxk;
-            R[0] += (nuR[0] * gR - nuL[0] * gL) / dxk;
-          }
-          // thermal energy code 173
-          {
-            const double gL = (qe0 - nb_qe[k][0]) / dxk;
-            const double gR = (nb_qe[k][1] - qe0) / dxk;
-            R[1 + NDIR] += (nuR[1] * gR - nuL[1] * gL) / dxk;
-          }
-          // momentum, i==k branch (twice for l==k)
-          const double rhoL = (rt0 + nb_rt[k][0]) / 2.0;
-          const double rhoR = (nb_rt[k][1] + rt0) / 2.0;
-#pragma unroll
-          for (int l = 0; l < NDIM; l++) {
-            const double v0 = p0.m[l] * inv0;
-            const double tL = nuL[2 + l] * ((v0 - nb_v[k][l][0]) / dxk);
-            const double tR = nuR[2 + l] * ((nb_v[k][l][1] - v0) / dxk);
-            const double mL = (p0.m[l] + nb_m[k][l][0]) / 2.0;
-            const double mR = (nb_m[k][l][1] + p0.m[l]) / 2.0;
-            const double dm = (rhoR * tR - rhoL * tL) / dxk / 2.0;
-            const double de = (mR * tR - mL * tL) / dxk / 2.0;
-            const double mult = (l == k) ? 2.0 : 1.0;
-            R[1 + l] += mult * dm;
-            R[1 + NDIR] += mult * de;
-          }
-          // magnetic field, m==k branch, l != k
-          int pb = 2 + NDIM;
-#pragma unroll
-          for (int l = 0; l < NDIM; l++) {
-            if (l == k) continue;
-            const double b0 = p0.b[l];
-            const double tL = nuL[pb] * ((b0 - nb_b[k][l][0]) / dxk);
-            const double tR = nuR[pb] * ((nb_b[k][l][1] - b0) / dxk);
-            const double BL = (b0 + nb_b[k][l][0]) / 2.0;
-            const double BR = (nb_b[k][l][1] + b0) / 2.0;
-            R[2 + NDIR + l] += (tR - tL) / dxk;
-            R[1 + NDIR] += (tR * BR - tL * BL) / dxk;
-            pb++;
-          }
-        }
-      }
-
-      // cross terms: diffusion along k, derivative along l (l != k)
-#pragma unroll
-      for (int k = 0; k < NDIM; k++) {
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) {
-          if (l == k) continue;
-          const bool zk = (c[k] == g.Mlo[k] && !g.per_lo[k]) || (c[k] == g.Mhi[k] && !g.per_hi[k]);
-          const bool zl = (c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l]);
-          if (zl) continue;  // gradient1 along l is zero at c
-          // pair indices: momentum m_l along k, field b_l along k
-          const int pm = 2 + l;
-          const int pbf = 2 + NDIM + (l < k ? l : l - 1);
-          double Tm[2], Tb[2], rts[2], mks[2], bks[2];
-#pragma unroll
-          for (int sd = 0; sd < 2; sd++) {
-            const int sl = sd ? 1 : -1;
-            const int cl = c[l] + sl;
-            rts[sd] = nb_rt[l][sd];
-            mks[sd] = nb_m[l][k][sd];
-            bks[sd] = nb_b[l][k][sd];
-            Tm[sd] = 0.0;
-            Tb[sd] = 0.0;
-            const bool inmesh = (cl >= g.Mlo[l]) && (cl <= g.Mhi[l]);
-            if (!inmesh || zk) continue;
-            const long long ip = ic + (long long)sl * g.st[l];
-            const long long ipp = ip + g.st[k];
-            const long long ipm = ip - g.st[k];
-            // v_l and b_l at the two diagonal neighbours
-            const double rtp = W[ipp] + B[fs + ipp];
-            const double rtm = W[ipm] + B[fs + ipm];
-            const double vlp = W[(1 + l) * fs + ipp] / rtp;
-            const double vlm = W[(1 + l) * fs + ipm] / rtm;
-            const double blp = W[(2 + NDIR + l) * fs + ipp];
-            const double blm = W[(2 + NDIR + l) * fs + ipm];
-            const double dxk = dxc[k];
-            const double Kv = ctot[k] * 0.4;
-            const double Kb = ctot[k] * 0.04;
-            const double nuLm = Kv * RT[(long long)(k * NP + pm) * fs + ip - g.st[k]] * dxk;
-            const double nuRm = Kv * RT[(long long)(k * NP + pm) * fs + ip] * dxk;
-            const double nuLb = Kb * RT[(long long)(k * NP + pbf) * fs + ip - g.st[k]] * dxk;
-            const double nuRb = Kb * RT[(long long)(k * NP + pbf) * fs + ip] * dxk;
-            Tm[sd] = (vlp - vlm) / dxk / 2.0 * (nuLm + nuRm) / 2.0 / 2.0;
-            Tb[sd] = (blp - blm) / dxk / 2.0 * (nuLb + nuRb) / 2.0;
-          }
-          const double dxl = dxc[l];
-          // m_k += d_l(rho_t T), e += d_l(m_k T)
-          R[1 + k] += (rts[1] * Tm[1] - rts[0] * Tm[0]) / dxl / 2.0;
-          R[1 + NDIR] += (mks[1] * Tm[1] - mks[0] * Tm[0]) / dxl / 2.0;
-          // b_k -= d_l(Tb), e -= d_l(Tb b_k)
-          R[2 + NDIR + k] -= (Tb[1] - Tb[0]) / dxl / 2.0;
-          R[1 + NDIR] -= (Tb[1] * bks[1] - Tb[0] * bks[0]) / dxl / 2.0;
+        for (int k = 0; k < NDIM; k++) {
+          if (k == l) continue;
+          const long long xi = (long long)cross_index(NDIM, k, l) * fs;
+          const double Tmp = inP ? a.T[xi + ip] : 0.0, Tmm = inM ? a.T[xi + im] : 0.0;
+          const double Tbp = inP ? a.T[xi + fs + ip] : 0.0, Tbm = inM ? a.T[xi + fs + im] : 0.0;
+          const double mkp = W[(1 + k) * fs + ip], mkm = W[(1 + k) * fs + im];
+          const double bkp = W[(2 + NDIR + k) * fs + ip], bkm = W[(2 + NDIR + k) * fs + im];
+          R[1 + k] += (rtp * Tmp - rtm * Tmm) * h;
+          R[1 + NDIR] += (mkp * Tmp - mkm * Tmm) * h;
+          R[2 + NDIR + k] -= (Tbp - Tbm) * h;
+          R[1 + NDIR] -= (Tbp * bkp - Tbm * bkm) * h;
         }
       }
     }
-    redv[2 * NDIM] = numax;
-
-    // ---- register update ----
+    // register update and cmax of the result
     Pt<NDIR> po;
     {
-      const double* __restrict__ base = a.base;
-      double* __restrict__ out = a.out;
       double vals[NE];
 #pragma unroll
       for (int v = 0; v < NE; v++) {
-        vals[v] = base[v * fs + ic] + a.qa * R[v];
-        out[v * fs + ic] = vals[v];
+        vals[v] = a.base[v * fs + ic] + a.qa * R[v];
+        a.out[v * fs + ic] = vals[v];
       }
       if (a.acc_out) {
 #pragma unroll
@@ -516,33 +512,23 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs a) {
       }
       po.rho = vals[0];
 #pragma unroll
-      for (int cdir = 0; cdir < NDIR; cdir++) {
-        po.m[cdir] = vals[1 + cdir];
-        po.b[cdir] = vals[2 + NDIR + cdir];
-        po.bg[cdir] = p0.bg[cdir];
+      for (int cd = 0; cd < NDIR; cd++) {
+        po.m[cd] = vals[1 + cd];
+        po.b[cd] = vals[2 + NDIR + cd];
+        po.bg[cd] = B[(2 + cd) * fs + ic];
       }
       po.e = vals[1 + NDIR];
-      po.eb = p0.eb;
-      po.rhob = p0.rhob;
+      po.eb = B[ic];
+      po.rhob = B[fs + ic];
     }
-    if (a.reduce_out) {
 #pragma unroll
-      for (int d = 0; d < NDIM; d++) {
-        const double cm = cmax_pt<NDIR>(po, d, gamma);
-        redv[d] = cm;
-        redv[NDIM + d] = cm / dxc[d];
-      }
+    for (int d = 0; d < NDIM; d++) {
+      const double cm = cmax_pt<NDIR>(po, d, a.gamma);
+      redv[d] = cm;
+      redv[NDIM + d] = cm / a.dx[d][c[d]];
     }
   }
-  // reductions (all threads take part)
-  {
-    double r1[2 * NDIM];
-#pragma unroll
-    for (int k = 0; k < 2 * NDIM; k++) r1[k] = redv[k];
-    if (a.reduce_out) block_max_atomic<2 * NDIM>(r1, a.red_out);
-    double r2[1] = {redv[2 * NDIM]};
-    if (a.visc_on) block_max_atomic<1>(r2, a.red_visc);
-  }
+  block_max_atomic<2 * NDIM>(redv, a.red_out);
 }
 
 // ---------------------------------------------------------------------
@@ -689,34 +675,32 @@ __global__ void k_axpy4(long long n, double* __restrict__ out, const double* a0,
 // launchers
 
 template <int NDIM, int NDIR>
-static void launch_all(int which, const void* args, hipStream_t st, long long nthreads) {
+static void launch_nd(int which, int axis, const void* args, hipStream_t st, long long nthreads) {
   const int bs = 256;
   const unsigned nb = (unsigned)((nthreads + bs - 1) / bs);
   if (nb == 0) return;
-  switch (which) {
-    case 0:
-      hipLaunchKernelGGL((k_ratio<NDIM, NDIR>), dim3(nb), dim3(bs), 0, st,
-                         *(const RatioArgs*)args);
-      break;
-    case 1:
-      hipLaunchKernelGGL((k_stage<NDIM, NDIR>), dim3(nb), dim3(bs), 0, st,
-                         *(const StageArgs*)args);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_cmax<NDIM, NDIR>), dim3(nb), dim3(bs), 0, st,
-                         *(const CmaxArgs*)args);
-      break;
+  if (which == 0) {
+    const AxisArgs& A = *(const AxisArgs*)args;
+    if (axis == 0) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0>), dim3(nb), dim3(bs), 0, st, A);
+    if constexpr (NDIM > 1)
+      if (axis == 1) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 1>), dim3(nb), dim3(bs), 0, st, A);
+    if constexpr (NDIM > 2)
+      if (axis == 2) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2>), dim3(nb), dim3(bs), 0, st, A);
+  } else if (which == 1) {
+    hipLaunchKernelGGL((k_final<NDIM, NDIR>), dim3(nb), dim3(bs), 0, st, *(const FinalArgs*)args);
+  } else {
+    hipLaunchKernelGGL((k_cmax<NDIM, NDIR>), dim3(nb), dim3(bs), 0, st, *(const CmaxArgs*)args);
   }
 }
 
-static void dispatch(int ndim, int ndir, int which, const void* args, hipStream_t st,
+static void dispatch(int ndim, int ndir, int which, int axis, const void* args, hipStream_t st,
                      long long n) {
-  if (ndim == 1 && ndir == 1) launch_all<1, 1>(which, args, st, n);
-  else if (ndim == 1 && ndir == 2) launch_all<1, 2>(which, args, st, n);
-  else if (ndim == 1 && ndir == 3) launch_all<1, 3>(which, args, st, n);
-  else if (ndim == 2 && ndir == 2) launch_all<2, 2>(which, args, st, n);
-  else if (ndim == 2 && ndir == 3) launch_all<2, 3>(which, args, st, n);
-  else if (ndim == 3 && ndir == 3) launch_all<3, 3>(which, args, st, n);
+  if (ndim == 1 && ndir == 1) launch_nd<1, 1>(which, axis, args, st, n);
+  else if (ndim == 1 && ndir == 2) launch_nd<1, 2>(which, axis, args, st, n);
+  else if (ndim == 1 && ndir == 3) launch_nd<1, 3>(which, axis, args, st, n);
+  else if (ndim == 2 && ndir == 2) launch_nd<2, 2>(which, axis, args, st, n);
+  else if (ndim == 2 && ndir == 3) launch_nd<2, 3>(which, axis, args, st, n);
+  else if (ndim == 3 && ndir == 3) launch_nd<3, 3>(which, axis, args, st, n);
 }
 
 static long long mesh_cells(const GridDesc& g) {
@@ -725,19 +709,16 @@ static long long mesh_cells(const GridDesc& g) {
   return n;
 }
 
-void launch_ratio(const RatioArgs& a, hipStream_t st) {
-  long long n = 1;
-  for (int d = 0; d < 3; d++)
-    n *= (d < a.g.ndim) ? (a.g.Mhi[d] - a.g.Mlo[d] + 2) : 1;
-  dispatch(a.g.ndim, a.g.ndir, 0, &a, st, n);
+void launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
+  dispatch(a.g.ndim, a.g.ndir, 0, axis, &a, st, mesh_cells(a.g));
 }
 
-void launch_stage(const StageArgs& a, hipStream_t st) {
-  dispatch(a.g.ndim, a.g.ndir, 1, &a, st, mesh_cells(a.g));
+void launch_final(const FinalArgs& a, hipStream_t st) {
+  dispatch(a.g.ndim, a.g.ndir, 1, 0, &a, st, mesh_cells(a.g));
 }
 
 void launch_cmax(const CmaxArgs& a, hipStream_t st) {
-  dispatch(a.g.ndim, a.g.ndir, 2, &a, st, mesh_cells(a.g));
+  dispatch(a.g.ndim, a.g.ndir, 2, 0, &a, st, mesh_cells(a.g));
 }
 
 void launch_ghost(const GhostArgs& a, hipStream_t st) {

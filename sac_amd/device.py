@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsac_amd.so")
 
 SLOT_W, SLOT_S1, SLOT_S2, SLOT_ACC = 0, 1, 2, 3
-K_STAGE, K_RATIO, K_GHOST, K_CMAX = 0, 1, 2, 3
+K_FINAL, K_AXIS, K_GHOST, K_CMAX, K_COPY = 0, 1, 2, 3, 4
 ADV = {"onestep": 1, "fourstep": 4}
 PREC = {"fp64": 0}
 
@@ -25,7 +25,7 @@ EXPORTS = [
     "sac_dev_init", "sac_dev_free", "sac_dev_upload", "sac_dev_download",
     "sac_dev_upload_slot", "sac_dev_download_slot", "sac_dev_getboundary",
     "sac_dev_getdt_courant", "sac_dev_getdt_special", "sac_dev_advance",
-    "sac_dev_advect1", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
+    "sac_dev_advect1", "sac_dev_copy_slot", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
     "sac_dev_timing", "sac_dev_timing_get", "sac_dev_synchronize",
     "sac_dev_memory", "sac_dev_last_error",
 ]
@@ -62,6 +62,7 @@ def load_library(path: str = LIB_PATH):
         "sac_dev_getdt_special": (I, [V, dp]),
         "sac_dev_advance": (I, [V, D]),
         "sac_dev_advect1": (I, [V, D, I, I]),
+        "sac_dev_copy_slot": (I, [V, I, I]),
         "sac_dev_get_maxviscoef": (I, [V, dp]),
         "sac_dev_set_maxviscoef": (I, [V, D]),
         "sac_dev_timing": (I, [V, I]),
@@ -196,6 +197,9 @@ class Device:
 
     def advect1(self, qdt, wct_slot, w_slot):
         _check(load_library().sac_dev_advect1(self._h, qdt, wct_slot, w_slot))
+
+    def copy_slot(self, dst, src):
+        _check(load_library().sac_dev_copy_slot(self._h, dst, src))
 
     @property
     def maxviscoef(self):

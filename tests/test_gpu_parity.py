@@ -120,13 +120,63 @@ def _to_2d(prob):
                             courantpar=0.3)
 
 
+def _gpu_run_prescribed(dev_mod, prob, dts):
+    """advance with a prescribed dt sequence (the oracle's)"""
+    dev = dev_mod.Device(prob)
+    dev.upload(prob.w)
+    dev.getboundary()
+    for dt in dts:
+        dev.advance(dt)
+    w = dev.download()
+    dev.close()
+    return w
+
+
 def test_briowu_steps(dev_mod, oracle_mod):
+    """Shock tubes: the dt of the reference is set by maxviscoef, which in
+    the exactly-constant regions of a shock tube is a ratio of rounding-level
+    differences; any change of rounding changes it (two builds of the oracle
+    itself diverge, tests/test_dt_conditioning.py).  The state update is
+    therefore compared under the oracle's own dt sequence."""
     prob = problems.brio_wu(n=800)
-    steps = 60
+    steps = 200
     ref, rdts, _ = oracle_run(prob, oracle_mod, steps)
-    got, gdts = _gpu_run(dev_mod, prob, steps)
-    np.testing.assert_allclose(gdts, rdts, rtol=1e-12)
+    got = _gpu_run_prescribed(dev_mod, prob, rdts)
     assert_close_state(prob, got, ref, TOL, 6, "brio-wu")
+
+
+@pytest.mark.parametrize("case", ["briowu", "strat3d"])
+def test_stage_maxviscoef_and_dt(dev_mod, oracle_mod, case):
+    """maxviscoef gathered by one stage, and the courant dt, on identical
+    inputs"""
+    if case == "briowu":
+        prob = problems.brio_wu(n=400)
+        nsteps = 3
+    else:
+        prob = problems.stratified(n=12, nxyz=(12, 10, 8))
+        nsteps = 2
+    o = make_oracle(prob, oracle_mod)
+    for _ in range(nsteps):
+        o.step(prob.courantpar)
+    w = o.get_w()
+    dev = dev_mod.Device(prob)
+    dev.upload(prob.w)
+    dev.getboundary()
+    dev.upload(w)
+    dtc_o, cm_o = o.getdt_courant(prob.courantpar)
+    dtc_g, cm_g = dev.getdt_courant(prob.courantpar)
+    assert cm_g == pytest.approx(cm_o, rel=1e-14)
+    o.maxviscoef = 0.0
+    ref = o.advect1(1e-5, w, w)
+    dev.upload_slot(dev_mod.SLOT_S1, w)
+    dev.upload_slot(dev_mod.SLOT_S2, w)
+    dev.maxviscoef = 0.0
+    dev.advect1(1e-5, dev_mod.SLOT_S1, dev_mod.SLOT_S2)
+    assert dev.maxviscoef == pytest.approx(o.maxviscoef, rel=1e-13)
+    dt_o = o.getdt_special(dtc_o)
+    dt_g = dev.getdt_special(dtc_g)
+    assert dt_g == pytest.approx(dt_o, rel=1e-13)
+    dev.close()
 
 
 def test_stratified_steps(dev_mod, oracle_mod):
@@ -140,7 +190,7 @@ def test_stratified_steps(dev_mod, oracle_mod):
 
 def test_onestep(dev_mod, oracle_mod):
     prob = problems.brio_wu(n=200)
-    prob.courantpar = 0.05
+    prob.courantpar = 0.05  # courant-limited: dt is well conditioned
     ref, rdts, _ = oracle_run(prob, oracle_mod, 20, typeadvance="onestep")
     got, gdts = _gpu_run(dev_mod, prob, 20, typeadvance="onestep")
     np.testing.assert_allclose(gdts, rdts, rtol=1e-12)
