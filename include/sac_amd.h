@@ -57,6 +57,9 @@ extern "C" {
 #define SAC_K_GHOST 2
 #define SAC_K_CMAX 3
 #define SAC_K_COPY 4
+#define SAC_K_AXIS1 5 /* k_axis launches of axis 1, 2, 3 separately */
+#define SAC_K_AXIS2 6
+#define SAC_K_AXIS3 7
 
 typedef void *sac_handle;
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,6 +41,7 @@ constexpr double kBigDouble = 1.0e99;
 
 struct TimedLaunch {
   int kernel;
+  int kernel2;  // per-axis id of a k_axis launch (-1 otherwise)
   hipEvent_t a, b;
 };
 
@@ -76,8 +78,9 @@ struct Dev {
   long long bytes = 0;
   bool timing = false;
   std::vector<TimedLaunch> timed;
-  double t_ms[5] = {0, 0, 0, 0, 0};
-  long long t_n[5] = {0, 0, 0, 0, 0};
+  double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long long t_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int xcd = 8;  // XCDs of the device (block renumbering of the line kernels)
 };
 
 long long fieldsz(const Dev& d) { return d.g.fs; }
@@ -102,12 +105,12 @@ void t_begin(Dev& d, int k, hipEvent_t* a) {
   hipEventRecord(*a, d.stream);
   (void)k;
 }
-void t_end(Dev& d, int k, hipEvent_t a) {
+void t_end(Dev& d, int k, hipEvent_t a, int k2 = -1) {
   if (!d.timing) return;
   hipEvent_t b;
   hipEventCreate(&b);
   hipEventRecord(b, d.stream);
-  d.timed.push_back({k, a, b});
+  d.timed.push_back({k, k2, a, b});
 }
 void t_collect(Dev& d) {
   for (auto& t : d.timed) {
@@ -116,6 +119,10 @@ void t_collect(Dev& d) {
     hipEventElapsedTime(&ms, t.a, t.b);
     d.t_ms[t.kernel] += ms;
     d.t_n[t.kernel] += 1;
+    if (t.kernel2 >= 0) {
+      d.t_ms[t.kernel2] += ms;
+      d.t_n[t.kernel2] += 1;
+    }
     hipEventDestroy(t.a);
     hipEventDestroy(t.b);
   }
@@ -167,10 +174,14 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   x.ctot_src = red_of(d, wct);
   x.red_visc = red_visc(d);
   for (int k = 0; k < d.ndim; k++) {
+    // axis 1 (rows of a plane): neighbour rows in one XCD's L2.  Axes 0 and 2
+    // keep the hardware order, whose round-robin already places the
+    // neighbour planes of a row in the same XCD.
+    x.xcd = (k == 1) ? d.xcd : 1;
     hipEvent_t e0{};
     t_begin(d, SAC_K_AXIS, &e0);
     sac::launch_axis(x, k, d.stream);
-    t_end(d, SAC_K_AXIS, e0);
+    t_end(d, SAC_K_AXIS, e0, 5 + k);
   }
   sac::launch_zero_u64(red_of(d, out), sac::kRedSlots, d.stream);
   sac::FinalArgs s{};
@@ -193,6 +204,7 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   }
   s.visc_on = d.visc_on;
   s.red_out = red_of(d, out);
+  s.xcd = 1;
   hipEvent_t e0{};
   t_begin(d, SAC_K_FINAL, &e0);
   sac::launch_final(s, d.stream);
@@ -621,7 +633,7 @@ int sac_dev_timing(sac_handle h, int enable) {
   Dev& d = *(Dev*)h;
   hipStreamSynchronize(d.stream);
   t_collect(d);
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < 8; k++) {
     d.t_ms[k] = 0.0;
     d.t_n[k] = 0;
   }
@@ -631,7 +643,7 @@ int sac_dev_timing(sac_handle h, int enable) {
 
 int sac_dev_timing_get(sac_handle h, int kernel, double* total_ms, long long* count) {
   if (int rc = check_handle(h)) return rc;
-  if (kernel < 0 || kernel > 4) return fail(SAC_E_ARG, "kernel id %d", kernel);
+  if (kernel < 0 || kernel > 7) return fail(SAC_E_ARG, "kernel id %d", kernel);
   Dev& d = *(Dev*)h;
   t_collect(d);
   if (total_ms) *total_ms = d.t_ms[kernel];

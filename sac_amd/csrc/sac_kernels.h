@@ -15,6 +15,7 @@ struct AxisArgs {
   int visc_on;
   const unsigned long long* ctot_src;  // c_tot per direction of wct
   unsigned long long* red_visc;        // maxviscoef accumulator
+  int xcd;                             // XCD count for block renumbering (1 = off)
 };
 
 struct FinalArgs {
@@ -34,6 +35,7 @@ struct FinalArgs {
   int grav_on[3];
   int visc_on;
   unsigned long long* red_out;  // cmax reductions of out
+  int xcd;
 };
 
 struct CmaxArgs {

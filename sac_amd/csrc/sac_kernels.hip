@@ -232,12 +232,23 @@ __device__ __forceinline__ int cross_index(int ndim, int k, int l) {
 //       Tm_{K,l} = d_K v_l * (nuL+nuR)_{m_l}/4,  Tb_{K,l} = d_K b_l * (nuL+nuR)_{b_l}/2.
 // The contribution is added to the rate accumulator R (written on K=0).
 
+// Workgroups are dealt to the 8 XCDs of the device in turn; renumbering them
+// so that each XCD takes one contiguous range of cells keeps the neighbour
+// rows/planes of a line stencil in the same XCD's L2.  The grid is a multiple
+// of 8 blocks, so the renumbering is a bijection.
+__device__ __forceinline__ long long xcd_block(int xcd) {
+  const unsigned nb = gridDim.x;
+  if (xcd <= 1 || (nb % (unsigned)xcd) != 0) return blockIdx.x;
+  const unsigned per = nb / (unsigned)xcd;
+  return (long long)(blockIdx.x % (unsigned)xcd) * per + blockIdx.x / (unsigned)xcd;
+}
+
 template <int NDIM, int NDIR, int K>
 __global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
   constexpr int NE = 2 + 2 * NDIR;
   constexpr int NP = 2 * NDIM + 1;
   const GridDesc& g = a.g;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = xcd_block(a.xcd) * blockDim.x + threadIdx.x;
   int lo[3], ext[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -438,7 +449,7 @@ template <int NDIM, int NDIR>
 __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
   constexpr int NE = 2 + 2 * NDIR;
   const GridDesc& g = a.g;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = xcd_block(a.xcd) * blockDim.x + threadIdx.x;
   int lo[3], ext[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -677,7 +688,8 @@ __global__ void k_axpy4(long long n, double* __restrict__ out, const double* a0,
 template <int NDIM, int NDIR>
 static void launch_nd(int which, int axis, const void* args, hipStream_t st, long long nthreads) {
   const int bs = 256;
-  const unsigned nb = (unsigned)((nthreads + bs - 1) / bs);
+  unsigned nb = (unsigned)((nthreads + bs - 1) / bs);
+  if (which <= 1) nb = (nb + 7) / 8 * 8;  // multiple of 8 for xcd_block
   if (nb == 0) return;
   if (which == 0) {
     const AxisArgs& A = *(const AxisArgs*)args;

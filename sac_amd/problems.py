@@ -22,6 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 BC = {"cont": 1, "fixed": 2, "periodic": 3, "symm": 4, "symm0": 5, "asymm": 6}
+BC_NAMES = {v: k for k, v in BC.items()}
 
 GRAV0 = 3  # grav0_ (1-based)
 NU = 6     # nu_ (1-based)
