@@ -31,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_STAGE = {"fp64": 296}
+# algorithmic bytes per mesh cell per RK stage: 4*n_evolve*s_e + n_background*8
+BYTES_PER_CELL_STAGE = {"fp64": 4 * 8 * 8 + 5 * 8, "mixed": 4 * 8 * 4 + 5 * 8}
 
 
 def parse():
@@ -42,6 +43,8 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="mesh cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision leg (FP32 evolving fields)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run the library's plain copy kernel on one register "
                          "(known bytes) for HBM-counter calibration")
@@ -70,23 +73,17 @@ def cpu_baseline(budget_s: float):
                       f"{el:.1f} s, C oracle (gcc -O2) on one host core"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
-              "supported (multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
-        return 2
+def timed_run(device, problems, n, steps, warmup, precision, calib_copy=False):
+    """Warm-up and timed RK4 steps of the stratified workload on one GPU.
+    Returns (wall seconds, kernel timings, final state, copy calibration)."""
     import numpy as np
-    from sac_amd import device, problems
-
-    n = args.n
     prob = problems.stratified(n=n)
-    dev = device.Device(prob)
+    dev = device.Device(prob, precision=precision)
     dev.upload(prob.w)
     dev.getboundary()
     del prob.w
     calib = None
-    if args.calib_copy:
+    if calib_copy:
         dev.timing(True)
         for _ in range(3):
             dev.copy_slot(device.SLOT_S2, device.SLOT_W)
@@ -94,37 +91,53 @@ def main():
         c_ms, c_n = dev.timing_get(device.K_COPY)
         dev.timing(False)
         G = n + 4
+        esz = 8 if precision == "fp64" else 4
         fs = (G + 15) // 16 * 16 * G * G  # padded field size (elements)
-        nbytes = 2 * 8 * 8 * fs           # 8 evolving fields read + written
+        nbytes = 2 * 8 * esz * fs          # 8 evolving fields read + written
         c_avg = c_ms / max(c_n, 1)
         calib = {"copy_avg_ms": round(c_avg, 4), "launches": c_n, "copy_bytes": nbytes,
                  "copy_GBs": round(nbytes / (c_avg / 1e3) / 1e9, 1)}
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         dev.step(0.3)
     dev.synchronize()
     dev.timing(True)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         dev.step(0.3)
     dev.synchronize()
     el = time.perf_counter() - t0
-    f_ms, f_n = dev.timing_get(device.K_FINAL)
-    a_ms, a_n = dev.timing_get(device.K_AXIS)
-    g_ms, g_n = dev.timing_get(device.K_GHOST)
+    tim = {k: dev.timing_get(getattr(device, k)) for k in
+           ("K_FINAL", "K_AXIS", "K_GHOST", "K_AXIS1", "K_AXIS2", "K_AXIS3")}
     dev.timing(False)
     w = dev.download()
-    finite = bool(np.isfinite(w).all())
     dev.close()
+    return el, tim, w, calib, bool(np.isfinite(w).all())
 
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    from sac_amd import device, problems
+    from tests.helpers import per_var_errors
+
+    n = args.n
+    el, tim, w64, calib, finite = timed_run(device, problems, n, args.steps, args.warmup,
+                                           "fp64", args.calib_copy)
+    f_ms, f_n = tim["K_FINAL"]
+    a_ms, a_n = tim["K_AXIS"]
+    g_ms, g_n = tim["K_GHOST"]
     ncell = n ** 3
     stages = args.steps * 4
     value = ncell * stages / el / 1e6
     # kernel time of one stage: 3 k_axis + 1 k_final + 1 k_ghost launches
     per_stage_ms = (a_ms + f_ms + g_ms) / max(f_n, 1)
-    avg_stage_kernel_s = per_stage_ms / 1e3
     alg_bytes = BYTES_PER_CELL_STAGE["fp64"] * ncell
-    achieved = alg_bytes / avg_stage_kernel_s / 1e9
+    achieved = alg_bytes / (per_stage_ms / 1e3) / 1e9
     stage_wall_s = el / stages
+    traffic = measured_traffic("fp64", n)
     out = {
         "metric": "Mcell-updates/s per RK stage at 256^3 (stratified perturbation MHD)",
         "value": round(value, 3),
@@ -143,11 +156,13 @@ def main():
                    "mesh": [n, n, n], "precision": "fp64", "typeadvance": "fourstep"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
                      "kernel": "RK stage = 3 x k_axis<3,3,K> + k_final<3,3> + k_ghost",
                      "avg_ms": round(per_stage_ms, 4),
                      "launches": f_n},
         "stage_breakdown_ms": {"k_axis": round(a_ms / max(a_n, 1), 4),
+                               "k_axis_per_axis": [round(tim[k][0] / max(tim[k][1], 1), 4)
+                                                   for k in ("K_AXIS1", "K_AXIS2", "K_AXIS3")],
                                "k_final": round(f_ms / max(f_n, 1), 4),
                                "k_ghost": round(g_ms / max(g_n, 1), 4),
                                "stage_wall": round(stage_wall_s * 1e3, 4)},
@@ -156,10 +171,38 @@ def main():
     }
     if calib is not None:
         out["copy_calibration"] = calib
+    if not args.no_mixed:
+        elm, timm, wm, _, finm = timed_run(device, problems, n, args.steps, args.warmup, "mixed")
+        prob = problems.stratified(n=n)
+        errs = per_var_errors(prob, wm, w64, 8)
+        fm, fmn = timm["K_FINAL"]
+        am, _ = timm["K_AXIS"]
+        gm, _ = timm["K_GHOST"]
+        st_ms = (am + fm + gm) / max(fmn, 1)
+        ach_m = BYTES_PER_CELL_STAGE["mixed"] * ncell / (st_ms / 1e3) / 1e9
+        out["forms"] = {"mixed": {
+            "value": round(ncell * stages / elm / 1e6, 3), "unit": "Mcell-updates/s",
+            "speedup_vs_fp64": round(el / elm, 3),
+            "rel_l2_vs_fp64_gpu": [float("%.3e" % e) for e in errs],
+            "after_steps": args.steps + args.warmup, "finite": finm,
+            "roofline": {"achieved": round(ach_m, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach_m / HBM_PEAK_GBS, 4),
+                         "bytes_per_cell_stage": BYTES_PER_CELL_STAGE["mixed"]}}}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
     return 0
+
+
+def measured_traffic(precision, n):
+    """HBM bytes per stage measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE of
+    the stage kernels, scaled by the copy-kernel calibration), from the
+    committed profile summary of this workload; None if absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_stratified{n}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_stage")
 
 
 if __name__ == "__main__":

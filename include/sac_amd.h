@@ -43,7 +43,10 @@ extern "C" {
 #define SAC_ADV_FOURSTEP 4
 
 /* precision of the evolving fields */
-#define SAC_PREC_FP64 0
+#define SAC_PREC_FP64 0  /* everything in FP64 */
+#define SAC_PREC_MIXED 1 /* evolving fields, fluxes and RK registers in FP32 with compensated
+                            flux differences; background fields, pressure recovery from the
+                            split energy, register update, cmax/dt and maxviscoef in FP64 */
 
 /* device registers addressed by the slot-level entry points */
 #define SAC_SLOT_W 0  /* the state w */
@@ -81,7 +84,8 @@ typedef void *sac_handle;
  *                   (regions ordered lower/upper per axis, src/vacgrid.t:53-69)
  *   fullgridini     1 if the ini file carried ghost cells
  *   typeadvance     SAC_ADV_FOURSTEP or SAC_ADV_ONESTEP
- *   precision       SAC_PREC_FP64
+ *   precision       SAC_PREC_FP64 or SAC_PREC_MIXED (the host arrays stay FP64
+ *                   either way; FP32 storage is converted at upload/download)
  */
 int sac_dev_init(int ndim, int ndir, const int *nx, int nw, const int *iws, int niws,
                  const double *eqpar, int neqpar, int grav0_, int nu_, const double *dx,

@@ -53,6 +53,8 @@ struct Dev {
   int nx[3] = {1, 1, 1};
   long long ncell_grid = 0;  // G1*G2*G3 (unpadded, host layout)
   int typeadvance = SAC_ADV_FOURSTEP;
+  int prec = SAC_PREC_FP64;
+  size_t esz = sizeof(double);  // bytes per element of the evolving fields
   int fullgridini = 0;
   double gamma = 0.0;
   double grav[3] = {0, 0, 0};
@@ -62,12 +64,12 @@ struct Dev {
   double dxmin[3] = {0, 0, 0};
   bool any_fixed = false;
   bool initialized = false;  // getboundary's 'initialized' (src/vacgrid.t:209)
-  // device buffers
-  double* reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+  // device buffers (evolving fields, R, T and snap in the storage precision)
+  void* reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   double* bg = nullptr;
-  double* rate = nullptr;   // R accumulator
-  double* cross = nullptr;  // cross-term factors T
-  double* snap = nullptr;   // 'fixed' values of the evolving fields
+  void* rate = nullptr;     // R accumulator
+  void* cross = nullptr;    // cross-term factors T
+  void* snap = nullptr;     // 'fixed' values of the evolving fields
   double* bgsnap = nullptr;
   double* dx[3] = {nullptr, nullptr, nullptr};
   int* bc_e = nullptr;      // [ne][6]
@@ -85,15 +87,20 @@ struct Dev {
 
 long long fieldsz(const Dev& d) { return d.g.fs; }
 
-int dalloc(Dev& d, double** p, long long nelem) {
-  hipError_t e = hipMalloc((void**)p, sizeof(double) * (size_t)nelem);
-  if (e != hipSuccess) return fail(SAC_E_NOMEM, "hipMalloc of %lld doubles failed: %s", nelem,
+int balloc(Dev& d, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(SAC_E_NOMEM, "hipMalloc of %zu bytes failed: %s", bytes,
                                    hipGetErrorString(e));
-  d.bytes += (long long)sizeof(double) * nelem;
-  e = hipMemset(*p, 0, sizeof(double) * (size_t)nelem);
+  d.bytes += (long long)bytes;
+  e = hipMemset(*p, 0, bytes);
   if (e != hipSuccess) return fail(SAC_E_HIP, "hipMemset: %s", hipGetErrorString(e));
   return 0;
 }
+int dalloc(Dev& d, double** p, long long nelem) {
+  return balloc(d, (void**)p, sizeof(double) * (size_t)nelem);
+}
+// nelem elements of the storage precision
+int ealloc(Dev& d, void** p, long long nelem) { return balloc(d, p, d.esz * (size_t)nelem); }
 
 unsigned long long* red_of(Dev& d, int slot) { return d.red + (size_t)slot * sac::kRedSlots; }
 unsigned long long* red_visc(Dev& d) { return d.red + (size_t)kSlots * sac::kRedSlots; }
@@ -132,6 +139,7 @@ void t_collect(Dev& d) {
 void ghost_fill_evolving(Dev& d, int slot) {
   sac::GhostArgs a{};
   a.g = d.g;
+  a.prec = d.prec;
   a.f = d.reg[slot];
   a.bc = d.bc_e;
   a.nvar = d.ne;
@@ -147,6 +155,7 @@ void cmax_reduce(Dev& d, int slot) {
   sac::launch_zero_u64(red_of(d, slot), sac::kRedSlots, d.stream);
   sac::CmaxArgs a{};
   a.g = d.g;
+  a.prec = d.prec;
   a.w = d.reg[slot];
   a.bg = d.bg;
   for (int k = 0; k < 3; k++) a.dx[k] = d.dx[k];
@@ -164,6 +173,7 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   if (!d.red_valid[wct]) cmax_reduce(d, wct);
   sac::AxisArgs x{};
   x.g = d.g;
+  x.prec = d.prec;
   x.wct = d.reg[wct];
   x.bg = d.bg;
   for (int k = 0; k < 3; k++) x.dx[k] = d.dx[k];
@@ -186,6 +196,7 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   sac::launch_zero_u64(red_of(d, out), sac::kRedSlots, d.stream);
   sac::FinalArgs s{};
   s.g = d.g;
+  s.prec = d.prec;
   s.wct = d.reg[wct];
   s.bg = d.bg;
   for (int k = 0; k < 3; k++) s.dx[k] = d.dx[k];
@@ -219,28 +230,50 @@ int check_handle(sac_handle h) {
   return 0;
 }
 
-int copy_in(Dev& d, double* dst, const double* src, int nfield) {
-  // src: nfield fields of G1*G2*G3 doubles; dst: padded layout
-  const size_t w = sizeof(double) * (size_t)d.g.G[0];
+// host (unpadded, FP64) <-> device (padded rows, element size esz) copies of
+// nfield fields; FP32 device fields are converted on the host, one field at
+// a time
+int copy_in(Dev& d, void* dst, const double* src, int nfield, size_t esz) {
+  const size_t w = esz * (size_t)d.g.G[0];
   const size_t spitch = w;
-  const size_t dpitch = sizeof(double) * (size_t)d.g.st[1];
+  const size_t dpitch = esz * (size_t)d.g.st[1];
   const size_t rows = (size_t)d.g.G[1] * (size_t)d.g.G[2];
+  std::vector<float> tmp;
   for (int f = 0; f < nfield; f++) {
-    HIPCHK(hipMemcpy2DAsync(dst + (size_t)f * d.g.fs, dpitch, src + (size_t)f * d.ncell_grid,
-                            spitch, w, rows, hipMemcpyHostToDevice, d.stream));
+    const double* s = src + (size_t)f * d.ncell_grid;
+    char* dp = (char*)dst + esz * (size_t)f * d.g.fs;
+    if (esz == sizeof(double)) {
+      HIPCHK(hipMemcpy2DAsync(dp, dpitch, s, spitch, w, rows, hipMemcpyHostToDevice, d.stream));
+    } else {
+      tmp.resize((size_t)d.ncell_grid);
+      for (long long i = 0; i < d.ncell_grid; i++) tmp[i] = (float)s[i];
+      HIPCHK(hipMemcpy2DAsync(dp, dpitch, tmp.data(), spitch, w, rows, hipMemcpyHostToDevice,
+                              d.stream));
+      HIPCHK(hipStreamSynchronize(d.stream));
+    }
   }
   HIPCHK(hipStreamSynchronize(d.stream));
   return 0;
 }
 
-int copy_out(Dev& d, double* dst, const double* src, int nfield) {
-  const size_t w = sizeof(double) * (size_t)d.g.G[0];
+int copy_out(Dev& d, double* dst, const void* src, int nfield, size_t esz) {
+  const size_t w = esz * (size_t)d.g.G[0];
   const size_t dpitch = w;
-  const size_t spitch = sizeof(double) * (size_t)d.g.st[1];
+  const size_t spitch = esz * (size_t)d.g.st[1];
   const size_t rows = (size_t)d.g.G[1] * (size_t)d.g.G[2];
+  std::vector<float> tmp;
   for (int f = 0; f < nfield; f++) {
-    HIPCHK(hipMemcpy2DAsync(dst + (size_t)f * d.ncell_grid, dpitch, src + (size_t)f * d.g.fs,
-                            spitch, w, rows, hipMemcpyDeviceToHost, d.stream));
+    double* o = dst + (size_t)f * d.ncell_grid;
+    const char* sp = (const char*)src + esz * (size_t)f * d.g.fs;
+    if (esz == sizeof(double)) {
+      HIPCHK(hipMemcpy2DAsync(o, dpitch, sp, spitch, w, rows, hipMemcpyDeviceToHost, d.stream));
+    } else {
+      tmp.resize((size_t)d.ncell_grid);
+      HIPCHK(hipMemcpy2DAsync(tmp.data(), dpitch, sp, spitch, w, rows, hipMemcpyDeviceToHost,
+                              d.stream));
+      HIPCHK(hipStreamSynchronize(d.stream));
+      for (long long i = 0; i < d.ncell_grid; i++) o[i] = (double)tmp[i];
+    }
   }
   HIPCHK(hipStreamSynchronize(d.stream));
   return 0;
@@ -286,7 +319,8 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   if (eqpar[1] != 0.0) return fail(SAC_E_UNSUPPORTED, "resistive MHD (eta /= 0) is not built");
   if (typeadvance != SAC_ADV_FOURSTEP && typeadvance != SAC_ADV_ONESTEP)
     return fail(SAC_E_UNSUPPORTED, "typeadvance code %d not built", typeadvance);
-  if (precision != SAC_PREC_FP64) return fail(SAC_E_UNSUPPORTED, "precision %d not built", precision);
+  if (precision != SAC_PREC_FP64 && precision != SAC_PREC_MIXED)
+    return fail(SAC_E_UNSUPPORTED, "precision %d not built", precision);
   for (int d = 0; d < ndim; d++)
     if (nx[d] < 6) return fail(SAC_E_ARG, "nx[%d]=%d < 6", d, nx[d]);
   for (int iw = 0; iw < nw; iw++)
@@ -316,6 +350,8 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   d->nbg = 2 + ndir;
   d->npairs = ndim * (2 * ndim + 1);
   d->typeadvance = typeadvance;
+  d->prec = precision;
+  d->esz = (precision == SAC_PREC_MIXED) ? sizeof(float) : sizeof(double);
   d->fullgridini = fullgridini;
   d->gamma = eqpar[0];
   sac::GridDesc& g = d->g;
@@ -366,14 +402,14 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   };
   if (hipStreamCreate(&d->stream) != hipSuccess) return bail(fail(SAC_E_HIP, "hipStreamCreate"));
   for (int s = 0; s < kSlots; s++)
-    if ((rc = dalloc(*d, &d->reg[s], (long long)ne * g.fs))) return bail(rc);
+    if ((rc = ealloc(*d, &d->reg[s], (long long)ne * g.fs))) return bail(rc);
   if ((rc = dalloc(*d, &d->bg, (long long)d->nbg * g.fs))) return bail(rc);
-  if ((rc = dalloc(*d, &d->rate, (long long)ne * g.fs))) return bail(rc);
+  if ((rc = ealloc(*d, &d->rate, (long long)ne * g.fs))) return bail(rc);
   if (d->visc_on && ndim > 1 &&
-      (rc = dalloc(*d, &d->cross, (long long)2 * ndim * (ndim - 1) * g.fs)))
+      (rc = ealloc(*d, &d->cross, (long long)2 * ndim * (ndim - 1) * g.fs)))
     return bail(rc);
   if (d->any_fixed) {
-    if ((rc = dalloc(*d, &d->snap, (long long)ne * g.fs))) return bail(rc);
+    if ((rc = ealloc(*d, &d->snap, (long long)ne * g.fs))) return bail(rc);
     if ((rc = dalloc(*d, &d->bgsnap, (long long)d->nbg * g.fs))) return bail(rc);
   }
   // dx per axis, dxmin per axis over the mesh
@@ -454,8 +490,8 @@ int sac_dev_upload(sac_handle h, const double* w) {
   if (int rc = check_handle(h)) return rc;
   if (!w) return fail(SAC_E_ARG, "w is null");
   Dev& d = *(Dev*)h;
-  if (int rc = copy_in(d, d.reg[SAC_SLOT_W], w, d.ne)) return rc;
-  if (int rc = copy_in(d, d.bg, w + (size_t)d.ne * d.ncell_grid, d.nbg)) return rc;
+  if (int rc = copy_in(d, d.reg[SAC_SLOT_W], w, d.ne, d.esz)) return rc;
+  if (int rc = copy_in(d, d.bg, w + (size_t)d.ne * d.ncell_grid, d.nbg, sizeof(double))) return rc;
   d.red_valid[SAC_SLOT_W] = false;
   return 0;
 }
@@ -464,8 +500,8 @@ int sac_dev_download(sac_handle h, double* w) {
   if (int rc = check_handle(h)) return rc;
   if (!w) return fail(SAC_E_ARG, "w is null");
   Dev& d = *(Dev*)h;
-  if (int rc = copy_out(d, w, d.reg[SAC_SLOT_W], d.ne)) return rc;
-  return copy_out(d, w + (size_t)d.ne * d.ncell_grid, d.bg, d.nbg);
+  if (int rc = copy_out(d, w, d.reg[SAC_SLOT_W], d.ne, d.esz)) return rc;
+  return copy_out(d, w + (size_t)d.ne * d.ncell_grid, d.bg, d.nbg, sizeof(double));
 }
 
 int sac_dev_upload_slot(sac_handle h, int slot, const double* w) {
@@ -473,15 +509,15 @@ int sac_dev_upload_slot(sac_handle h, int slot, const double* w) {
   if (slot < 0 || slot >= kSlots || !w) return fail(SAC_E_ARG, "bad slot %d", slot);
   Dev& d = *(Dev*)h;
   d.red_valid[slot] = false;
-  return copy_in(d, d.reg[slot], w, d.ne);
+  return copy_in(d, d.reg[slot], w, d.ne, d.esz);
 }
 
 int sac_dev_download_slot(sac_handle h, int slot, double* w) {
   if (int rc = check_handle(h)) return rc;
   if (slot < 0 || slot >= kSlots || !w) return fail(SAC_E_ARG, "bad slot %d", slot);
   Dev& d = *(Dev*)h;
-  if (int rc = copy_out(d, w, d.reg[slot], d.ne)) return rc;
-  return copy_out(d, w + (size_t)d.ne * d.ncell_grid, d.bg, d.nbg);
+  if (int rc = copy_out(d, w, d.reg[slot], d.ne, d.esz)) return rc;
+  return copy_out(d, w + (size_t)d.ne * d.ncell_grid, d.bg, d.nbg, sizeof(double));
 }
 
 int sac_dev_getboundary(sac_handle h) {
@@ -491,6 +527,7 @@ int sac_dev_getboundary(sac_handle h) {
     // first call: background ghosts too, then store 'fixed' values
     sac::GhostArgs a{};
     a.g = d.g;
+    a.prec = sac::PREC_FP64;
     a.f = d.bg;
     a.bc = d.bc_b;
     a.nvar = d.nbg;
@@ -499,7 +536,7 @@ int sac_dev_getboundary(sac_handle h) {
     sac::launch_ghost(a, d.stream);
     ghost_fill_evolving(d, SAC_SLOT_W);
     if (d.any_fixed) {
-      HIPCHK(hipMemcpyAsync(d.snap, d.reg[SAC_SLOT_W], sizeof(double) * (size_t)d.ne * d.g.fs,
+      HIPCHK(hipMemcpyAsync(d.snap, d.reg[SAC_SLOT_W], d.esz * (size_t)d.ne * d.g.fs,
                             hipMemcpyDeviceToDevice, d.stream));
       HIPCHK(hipMemcpyAsync(d.bgsnap, d.bg, sizeof(double) * (size_t)d.nbg * d.g.fs,
                             hipMemcpyDeviceToDevice, d.stream));
@@ -566,7 +603,7 @@ int sac_dev_advance(sac_handle h, double dt) {
     stage(d, S1, A, W, dt / 6, -1, -1, 0.0);
   } else {
     // onestep: w1=w; advect1(dt, wCT=w1, w) (src/vac.t:407-408)
-    HIPCHK(hipMemcpyAsync(d.reg[S1], d.reg[W], sizeof(double) * (size_t)d.ne * d.g.fs,
+    HIPCHK(hipMemcpyAsync(d.reg[S1], d.reg[W], d.esz * (size_t)d.ne * d.g.fs,
                           hipMemcpyDeviceToDevice, d.stream));
     d.red_valid[S1] = d.red_valid[W];
     if (d.red_valid[W])
@@ -598,8 +635,8 @@ int sac_dev_copy_slot(sac_handle h, int dst_slot, int src_slot) {
   if (dst_slot == src_slot) return 0;
   hipEvent_t e0{};
   t_begin(d, SAC_K_COPY, &e0);
-  sac::launch_axpy4((long long)d.ne * d.g.fs, d.reg[dst_slot], d.reg[src_slot], 1.0, nullptr, 0.0,
-                    nullptr, 0.0, nullptr, 0.0, d.stream);
+  sac::launch_axpy4(d.prec, (long long)d.ne * d.g.fs, d.reg[dst_slot], d.reg[src_slot], 1.0,
+                    nullptr, 0.0, nullptr, 0.0, nullptr, 0.0, d.stream);
   t_end(d, SAC_K_COPY, e0);
   d.red_valid[dst_slot] = false;
   HIPCHK(hipGetLastError());

@@ -18,6 +18,9 @@ constexpr int kMaxBg = 2 + kMaxDir;          // eb, rhob, bg(3)
 constexpr int kMaxPairs = 21;                // setnu (variable, direction) pairs
 constexpr int kRedSlots = 8;                 // reduction values per buffer
 
+// storage precision of the evolving fields
+enum PrecCode : int { PREC_FP64 = 0, PREC_MIXED = 1 };
+
 // boundary codes (typeB strings, src/vacgrid.t:257-464)
 enum BcCode : int { BC_CONT = 1, BC_FIXED = 2, BC_PERIODIC = 3, BC_SYMM = 4, BC_SYMM0 = 5, BC_ASYMM = 6 };
 

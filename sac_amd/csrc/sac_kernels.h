@@ -4,13 +4,17 @@
 
 namespace sac {
 
+// Pointers to evolving fields, R and T have the storage type of the
+// precision (double for PREC_FP64, float for PREC_MIXED); the background and
+// dx are always double.
 struct AxisArgs {
   GridDesc g;
-  const double* wct;     // stage input (time-centred state)
+  int prec;
+  const void* wct;       // stage input (time-centred state)
   const double* bg;      // background fields
   const double* dx[3];   // per-axis cell widths, indexed by grid index
-  double* R;             // rate accumulator (n_evolve fields)
-  double* T;             // cross-term factors (2*ndim*(ndim-1) fields)
+  void* R;               // rate accumulator (n_evolve fields)
+  void* T;               // cross-term factors (2*ndim*(ndim-1) fields)
   double gamma;
   int visc_on;
   const unsigned long long* ctot_src;  // c_tot per direction of wct
@@ -20,15 +24,16 @@ struct AxisArgs {
 
 struct FinalArgs {
   GridDesc g;
-  const double* wct;
+  int prec;
+  const void* wct;
   const double* bg;
   const double* dx[3];
-  const double* R;
-  const double* T;
-  const double* base;    // out = base + qa*R
-  double* out;
-  const double* acc_in;  // acc_out = acc_in + qb*R (acc_out may be null)
-  double* acc_out;
+  const void* R;
+  const void* T;
+  const void* base;      // out = base + qa*R
+  void* out;
+  const void* acc_in;    // acc_out = acc_in + qb*R (acc_out may be null)
+  void* acc_out;
   double qa, qb;
   double gamma;
   double grav[3];
@@ -40,7 +45,8 @@ struct FinalArgs {
 
 struct CmaxArgs {
   GridDesc g;
-  const double* w;
+  int prec;
+  const void* w;
   const double* bg;
   const double* dx[3];
   double gamma;
@@ -49,8 +55,9 @@ struct CmaxArgs {
 
 struct GhostArgs {
   GridDesc g;
-  double* f;           // nvar fields to fill in place
-  const double* snap;  // stored 'fixed' values (null before they exist)
+  int prec;            // storage type of f and snap
+  void* f;             // nvar fields to fill in place
+  const void* snap;    // stored 'fixed' values (null before they exist)
   const int* bc;       // [nvar][6] boundary codes
   int nvar;
   int keep_fixed;      // initial fill with fullgridini: keep file values
@@ -61,8 +68,10 @@ void launch_final(const FinalArgs& a, hipStream_t st);
 void launch_cmax(const CmaxArgs& a, hipStream_t st);
 void launch_ghost(const GhostArgs& a, hipStream_t st);
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
-void launch_axpy4(long long n, double* out, const double* a0, double c0, const double* a1,
-                  double c1, const double* a2, double c2, const double* a3, double c3,
+void launch_axpy4(int prec, long long n, void* out, const void* a0, double c0, const void* a1,
+                  double c1, const void* a2, double c2, const void* a3, double c3,
                   hipStream_t st);
+void launch_to_float(long long n, float* out, const double* in, hipStream_t st);
+void launch_to_double(long long n, double* out, const float* in, hipStream_t st);
 
 }  // namespace sac

@@ -1,8 +1,16 @@
 // sac_kernels.hip -- HIP kernels (gfx950) of the SAC explicit
-// perturbation-MHD stage: face ratios of the hyperviscosity coefficient,
-// the fused stage update (flux divergence, background energy terms,
-// gravity, hyperdiffusion, Runge-Kutta register update and the cmax /
-// maxviscoef reductions of the result), ghost-cell fill and reductions.
+// perturbation-MHD stage: per-axis line work (fluxes, setnu, hyperdiffusion),
+// the combining update (cross terms, gravity, Runge-Kutta register update and
+// the cmax reductions of the result), ghost-cell fill and reductions.
+//
+// Every kernel is a template over the storage type S of the evolving fields
+// (and of the rate and cross-term scratch) and the compute type C of the
+// flux and hyperdiffusion arithmetic:
+//   FP64:  S = C = double;
+//   mixed: S = C = float, with FP64 kept for the background fields, the
+//          pressure recovery from the split total energy (getptotal and the
+//          thermal energy of setnu), the register update, cmax and every
+//          reduction; the flux differences of gradient4 are compensated.
 //
 // Reference routines restated here:
 //   getflux            src/vacphys.mhd0.t:147-222
@@ -17,6 +25,14 @@
 //   getboundary        src/vacgrid.t:195-512
 #include "sac_kernels.h"
 
+// mesh cells per thread of the line kernels along x and along y/z
+#ifndef SAC_CK_X
+#define SAC_CK_X 2
+#endif
+#ifndef SAC_CK_YZ
+#define SAC_CK_YZ 2
+#endif
+
 // keeps the machine scheduler from hoisting the next phase's loads above
 // the current phase's arithmetic, which bounds the number of live values
 #define SCHED_FENCE()                  \
@@ -28,86 +44,136 @@
 namespace sac {
 
 // ---------------------------------------------------------------------
-// point state
+// precision helpers
 
-template <int NDIR>
+// a/b for 0 <= a <= 6b, b > 0 (the setnu ratio md3/md1): IEEE division in
+// FP64; in FP32 a reciprocal-multiply, with both operands scaled by 2^100
+// (exact) when b is below 2^-100 so that the reciprocal of a denormal never
+// overflows
+__device__ __forceinline__ double cdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float cdiv(float a, float b) {
+  const float s = (b < 0x1p-100f) ? 0x1p100f : 1.0f;
+  return (a * s) * __builtin_amdgcn_rcpf(b * s);
+}
+
+// 1/x in FP64 for the pressure recovery: IEEE division in FP64 mode; in
+// mixed mode the FP32 reciprocal refined by one Newton step in FP64
+// (relative error ~1e-14)
+template <typename C>
+__device__ __forceinline__ double recip(double x);
+template <>
+__device__ __forceinline__ double recip<double>(double x) {
+  return 1.0 / x;
+}
+template <>
+__device__ __forceinline__ double recip<float>(double x) {
+  const double r = (double)__builtin_amdgcn_rcpf((float)x);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+
+// a-b with its rounding error (FP64 mode: error 0, folded away)
+__device__ __forceinline__ double two_diff(double a, double b, double& err) {
+  err = 0.0;
+  return a - b;
+}
+__device__ __forceinline__ float two_diff(float a, float b, float& err) {
+  const float s = a - b;
+  const float bb = s - a;
+  err = (a - (s - bb)) - (b + bb);
+  return s;
+}
+
+// ---------------------------------------------------------------------
+// point state: perturbation (evolving) in C, static background in FP64
+
+template <int NDIR, typename C>
 struct Pt {
-  double rho, m[NDIR], e, b[NDIR];  // perturbation (evolving)
-  double eb, rhob, bg[NDIR];        // static background
+  C rho, m[NDIR], e, b[NDIR];
+  double eb, rhob, bg[NDIR];
 };
 
-template <int NDIR>
-__device__ __forceinline__ void load_pt(const double* __restrict__ W, const double* __restrict__ B,
-                                        long long fs, long long i, Pt<NDIR>& p) {
-  p.rho = W[i];
+template <int NDIR, typename S, typename C>
+__device__ __forceinline__ void load_pt(const S* __restrict__ W, const double* __restrict__ B,
+                                        long long fs, long long i, Pt<NDIR, C>& p) {
+  p.rho = (C)W[i];
 #pragma unroll
-  for (int c = 0; c < NDIR; c++) p.m[c] = W[(1 + c) * fs + i];
-  p.e = W[(1 + NDIR) * fs + i];
+  for (int c = 0; c < NDIR; c++) p.m[c] = (C)W[(1 + c) * fs + i];
+  p.e = (C)W[(1 + NDIR) * fs + i];
 #pragma unroll
-  for (int c = 0; c < NDIR; c++) p.b[c] = W[(2 + NDIR + c) * fs + i];
+  for (int c = 0; c < NDIR; c++) p.b[c] = (C)W[(2 + NDIR + c) * fs + i];
   p.eb = B[i];
   p.rhob = B[fs + i];
 #pragma unroll
   for (int c = 0; c < NDIR; c++) p.bg[c] = B[(2 + c) * fs + i];
 }
 
-// perturbation total pressure p_T' (getptotal, src/vacphys0.t.mhd:186-205)
-template <int NDIR>
-__device__ __forceinline__ double ptotal(const Pt<NDIR>& p, double gamma, double inv) {
+// perturbation total pressure p_T' in FP64 (getptotal, src/vacphys0.t.mhd:186-205);
+// inv = 1/rho_t
+template <int NDIR, typename C>
+__device__ __forceinline__ double ptotal(const Pt<NDIR, C>& p, double gamma, double inv) {
   double sbbg = 0.0, sb2 = 0.0, sm2 = 0.0;
 #pragma unroll
   for (int c = 0; c < NDIR; c++) {
-    sbbg += p.b[c] * p.bg[c];
-    sb2 += p.b[c] * p.b[c];
-    sm2 += p.m[c] * p.m[c];
+    const double b = (double)p.b[c], m = (double)p.m[c];
+    sbbg += b * p.bg[c];
+    sb2 += b * b;
+    sm2 += m * m;
   }
-  return (gamma - 1.0) * (p.e - 0.5 * sm2 * inv) - (gamma - 2.0) * (sbbg + 0.5 * sb2);
+  return (gamma - 1.0) * ((double)p.e - 0.5 * sm2 * inv) - (gamma - 2.0) * (sbbg + 0.5 * sb2);
 }
 
 // background total pressure (getptotal_bg, src/vacphys0.t.mhd:208-224)
-template <int NDIR>
-__device__ __forceinline__ double ptotal_bg(const Pt<NDIR>& p, double gamma) {
+template <int NDIR, typename C>
+__device__ __forceinline__ double ptotal_bg(const Pt<NDIR, C>& p, double gamma) {
   double s = 0.0;
 #pragma unroll
   for (int c = 0; c < NDIR; c++) s += p.bg[c] * p.bg[c];
   return (gamma - 1.0) * p.eb - 0.5 * (gamma - 2.0) * s;
 }
 
-// full flux (non-transport part of getflux + transport v_d*w) along axis d
-template <int NDIR>
-__device__ __forceinline__ void flux_pt(const Pt<NDIR>& p, int d, double gamma,
-                                        double (&F)[2 + 2 * NDIR]) {
-  const double rt = p.rho + p.rhob;
-  const double inv = 1.0 / rt;
-  const double vd = p.m[d] * inv;
-  F[0] = p.rhob * p.m[d] * inv + vd * p.rho;
-  const double pT = ptotal<NDIR>(p, gamma, inv);
-  double sbgm = 0.0, sbm = 0.0;
+// full flux (non-transport part of getflux + transport v_d*w) along axis d;
+// also returns v_l = m_l/rho_t for l < NV
+template <int NDIR, int NV, typename C>
+__device__ __forceinline__ void flux_pt(const Pt<NDIR, C>& p, int d, double gamma,
+                                        C (&F)[2 + 2 * NDIR], C (&v)[NV]) {
+  const double rt = (double)p.rho + p.rhob;
+  const double invd = recip<C>(rt);
+  const C inv = (C)invd;
+  const C vd = p.m[d] * inv;
+  F[0] = (C)p.rhob * p.m[d] * inv + vd * p.rho;
+  const C pT = (C)ptotal<NDIR, C>(p, gamma, invd);
+  C sbgm = 0, sbm = 0;
 #pragma unroll
   for (int c = 0; c < NDIR; c++) {
-    sbgm += p.bg[c] * p.m[c];
+    sbgm += (C)p.bg[c] * p.m[c];
     sbm += p.b[c] * p.m[c];
   }
 #pragma unroll
   for (int c = 0; c < NDIR; c++) {
-    double f = -(p.b[c] * p.bg[d] + p.b[d] * p.bg[c]) - p.b[c] * p.b[d];
+    C f = -(p.b[c] * (C)p.bg[d] + p.b[d] * (C)p.bg[c]) - p.b[c] * p.b[d];
     if (c == d) f = pT + f;
     F[1 + c] = f + vd * p.m[c];
   }
-  F[1 + NDIR] = (p.m[d] * pT - p.b[d] * sbgm - p.bg[d] * sbm) * inv + p.eb * p.m[d] * inv -
-                p.b[d] * sbm * inv + vd * p.e;
+  F[1 + NDIR] = (p.m[d] * pT - p.b[d] * sbgm - (C)p.bg[d] * sbm) * inv +
+                (C)p.eb * p.m[d] * inv - p.b[d] * sbm * inv + vd * p.e;
 #pragma unroll
   for (int c = 0; c < NDIR; c++) {
-    F[2 + NDIR + c] = (c == d) ? 0.0
-                               : (-p.m[c] * inv * (p.b[d] + p.bg[d]) + p.m[d] * inv * p.bg[c] +
-                                  vd * p.b[c]);
+    F[2 + NDIR + c] = (c == d) ? (C)0
+                               : (-p.m[c] * inv * (p.b[d] + (C)p.bg[d]) +
+                                  p.m[d] * inv * (C)p.bg[c] + vd * p.b[c]);
   }
+#pragma unroll
+  for (int l = 0; l < NV; l++) v[l] = p.m[l] * inv;
 }
 
-// fast-mode speed + |v_d| (getcmax, src/vacphys0.t.mhd:77-112)
-template <int NDIR>
-__device__ __forceinline__ double cmax_pt(const Pt<NDIR>& p, int d, double gamma) {
+// fast-mode speed + |v_d| (getcmax, src/vacphys0.t.mhd:77-112) in FP64 for
+// every direction d < NDIM; the direction-independent part (c_s^2, c_f^2,
+// 1/rho_t) is evaluated once
+template <int NDIM, int NDIR>
+__device__ __forceinline__ void cmax_all(const Pt<NDIR, double>& p, double gamma,
+                                         double (&cm)[NDIM]) {
   const double rt = p.rho + p.rhob;
+  const double inv = 1.0 / rt;
   double sm2 = 0.0, sb2 = 0.0, sbbg = 0.0, sbg2 = 0.0, sbt2 = 0.0;
 #pragma unroll
   for (int c = 0; c < NDIR; c++) {
@@ -118,13 +184,17 @@ __device__ __forceinline__ double cmax_pt(const Pt<NDIR>& p, int d, double gamma
     double bt = p.b[c] + p.bg[c];
     sbt2 += bt * bt;
   }
-  double pth = 0.5 * sm2 / rt;
+  double pth = 0.5 * sm2 * inv;
   pth = pth + 0.5 * sb2 + sbbg;
   pth = (gamma - 1.0) * (p.e - pth);
-  const double cs2 = gamma * (pth + (gamma - 1.0) * (p.eb - 0.5 * sbg2)) / rt;
-  const double cf2 = sbt2 / rt + cs2;
-  const double bd = p.b[d] + p.bg[d];
-  return sqrt(0.5 * (cf2 + sqrt(cf2 * cf2 - 4.0 * cs2 * (bd * bd) / rt))) + fabs(p.m[d] / rt);
+  const double cs2 = gamma * (pth + (gamma - 1.0) * (p.eb - 0.5 * sbg2)) * inv;
+  const double cf2 = sbt2 * inv + cs2;
+  const double cf4 = cf2 * cf2, cs4 = 4.0 * cs2 * inv;
+#pragma unroll
+  for (int d = 0; d < NDIM; d++) {
+    const double bd = p.b[d] + p.bg[d];
+    cm[d] = sqrt(0.5 * (cf2 + sqrt(cf4 - cs4 * (bd * bd)))) + fabs(p.m[d] * inv);
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -171,56 +241,38 @@ __device__ __forceinline__ long long lin(const GridDesc& g, int i0, int i1, int 
   return (long long)i0 * g.st[0] + (long long)i1 * g.st[1] + (long long)i2 * g.st[2];
 }
 
-// ---------------------------------------------------------------------
-// setnu variables at one point (q of setnu for every variable)
-
-template <int NDIR>
-struct Qv {
-  double rho, qe, rt, inv, v[3], m[3], b[3];
-};
-
-template <int NDIM, int NDIR>
-__device__ __forceinline__ void load_qv(const double* __restrict__ W, const double* __restrict__ B,
-                                        long long fs, long long i, Qv<NDIR>& q) {
-  q.rho = W[i];
-  q.rt = q.rho + B[fs + i];
-  q.inv = 1.0 / q.rt;
-  double sb = 0.0, sm = 0.0;
-#pragma unroll
-  for (int c = 0; c < NDIR; c++) {
-    const double mc = W[(1 + c) * fs + i];
-    const double bc = W[(2 + NDIR + c) * fs + i];
-    sb += bc * bc;
-    sm += mc * mc;
-    if (c < NDIM) {
-      q.m[c] = mc;
-      q.v[c] = mc * q.inv;
-      q.b[c] = bc;
-    }
-  }
-  q.qe = W[(1 + NDIR) * fs + i] - 0.5 * (sb + sm * q.inv);
-}
-
 // setnu ratio md3/md1 at the face between q[j] and q[j+1] of a 6-value
 // window q[j-2..j+3] (src/vacusr.viscosity.t:413-427)
-__device__ __forceinline__ double face_ratio(const double* q) {
-  double d1[5], d3[3];
+template <typename C>
+__device__ __forceinline__ C face_ratio(const C* q) {
+  C d1[5], d3[3];
 #pragma unroll
   for (int j = 0; j < 5; j++) d1[j] = fabs(q[j + 1] - q[j]);
 #pragma unroll
-  for (int j = 0; j < 3; j++) d3[j] = fabs(3.0 * (q[j + 2] - q[j + 1]) - (q[j + 3] - q[j]));
-  const double md3 = fmax(fmax(d3[0], d3[1]), d3[2]);
-  const double md1 = fmax(fmax(fmax(d1[0], d1[1]), fmax(d1[2], d1[3])), d1[4]);
-  return (md1 > 0.0) ? md3 / md1 : 0.0;
+  for (int j = 0; j < 3; j++) d3[j] = fabs((C)3 * (q[j + 2] - q[j + 1]) - (q[j + 3] - q[j]));
+  const C md3 = fmax(fmax(d3[0], d3[1]), d3[2]);
+  const C md1 = fmax(fmax(fmax(d1[0], d1[1]), fmax(d1[2], d1[3])), d1[4]);
+  return (md1 > (C)0) ? cdiv(md3, md1) : (C)0;
 }
 
 __device__ __forceinline__ int cross_index(int ndim, int k, int l) {
   return 2 * (k * (ndim - 1) + (l < k ? l : l - 1));
 }
 
+// Workgroups are dealt to the 8 XCDs of the device in turn; renumbering them
+// so that each XCD takes one contiguous range of cells keeps the neighbour
+// rows of a line stencil in the same XCD's L2.  The grid is a multiple of 8
+// blocks, so the renumbering is a bijection.
+__device__ __forceinline__ long long xcd_block(int xcd) {
+  const unsigned nb = gridDim.x;
+  if (xcd <= 1 || (nb % (unsigned)xcd) != 0) return blockIdx.x;
+  const unsigned per = nb / (unsigned)xcd;
+  return (long long)(blockIdx.x % (unsigned)xcd) * per + blockIdx.x / (unsigned)xcd;
+}
+
 // ---------------------------------------------------------------------
-// kernel 1: the work of one axis K for every mesh cell, all of which lies
-// on the K-line through the cell:
+// kernel 1: the work of one axis K for a group of CK consecutive mesh cells
+// along K, all of which lies on the K-line through the group:
 //   - 4th-order flux divergence along K (centdiff4, src/vaccd.t:35-70)
 //     and the background energy terms with derivatives along K,
 //   - setnu (src/vacusr.viscosity.t:224-455) for the 2*ndim+1 variables of
@@ -230,23 +282,243 @@ __device__ __forceinline__ int cross_index(int ndim, int k, int l) {
 //   - the cross-term factors T (diffusion along K) that kernel 2
 //     differentiates along the other axes:
 //       Tm_{K,l} = d_K v_l * (nuL+nuR)_{m_l}/4,  Tb_{K,l} = d_K b_l * (nuL+nuR)_{b_l}/2.
-// The contribution is added to the rate accumulator R (written on K=0).
+// The group shares its setnu work: the q values of the CK+6 stencil points
+// and the CK+1 face ratios are evaluated once for the CK cells.  The
+// contribution is added to the rate accumulator R (written on K=0).
 
-// Workgroups are dealt to the 8 XCDs of the device in turn; renumbering them
-// so that each XCD takes one contiguous range of cells keeps the neighbour
-// rows/planes of a line stencil in the same XCD's L2.  The grid is a multiple
-// of 8 blocks, so the renumbering is a bijection.
-__device__ __forceinline__ long long xcd_block(int xcd) {
-  const unsigned nb = gridDim.x;
-  if (xcd <= 1 || (nb % (unsigned)xcd) != 0) return blockIdx.x;
-  const unsigned per = nb / (unsigned)xcd;
-  return (long long)(blockIdx.x % (unsigned)xcd) * per + blockIdx.x / (unsigned)xcd;
-}
+template <int K>
+struct LineGroup {
+  static constexpr int CK = (K == 0) ? SAC_CK_X : SAC_CK_YZ;
+};
 
-template <int NDIM, int NDIR, int K>
-__global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
+// One group of CK consecutive mesh cells along K starting at c[K] (other
+// coordinates c[d]) of the grid g; adds the group's maximum nu to numax.
+template <int NDIM, int NDIR, int K, int CK, typename S, typename C>
+__device__ __forceinline__ void axis_group(const AxisArgs& a, const GridDesc& g, const int (&c)[3],
+                                           C& numax) {
   constexpr int NE = 2 + 2 * NDIR;
   constexpr int NP = 2 * NDIM + 1;
+  const int c0 = c[K];
+  const long long fs = g.fs;
+  const S* __restrict__ W = (const S*)a.wct;
+  const double* __restrict__ B = a.bg;
+  const double gamma = a.gamma;
+  const long long ic0 = lin(g, c[0], c[1], c[2]);
+  const long long sk = g.st[K];
+  const int GK = g.G[K], MloK = g.Mlo[K], MhiK = g.Mhi[K];
+  // per-cell geometry; cells beyond the mesh are computed on clamped
+  // indices and never read back
+  bool valid[CK], fluxc[CK], z1[CK];
+  C dxk[CK], idx[CK];
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    const int cj = c0 + j;
+    valid[j] = cj <= MhiK;
+    fluxc[j] = valid[j] && cj >= MloK + 2 && cj <= MhiK - 2;
+    z1[j] = (cj == MloK && !g.per_lo[K]) || (cj == MhiK && !g.per_hi[K]);
+    const double dxd = a.dx[K][cj < GK ? cj : GK - 1];
+    dxk[j] = (C)dxd;
+    idx[j] = (C)(1.0 / dxd);
+  }
+  C R[CK][NE];
+
+  // ---- flux divergence along K (zero on two layers at each side) ----
+  // cell by cell: the four neighbour fluxes of a cell (the neighbours are
+  // shared with the next cells through the caches).  Branch-free: a cell
+  // outside the flux range is evaluated on an in-grid cell and its result
+  // replaced by zero.
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    const int cf = (c0 + j > MhiK) ? MhiK : c0 + j;
+    const long long ic = ic0 + (long long)(cf - c0) * sk;
+    C bgk, bgl[NDIM], pTb0;
+    {
+      Pt<NDIR, C> p0;
+      load_pt<NDIR, S, C>(W, B, fs, ic, p0);
+      pTb0 = (C)ptotal_bg<NDIR, C>(p0, gamma);
+      bgk = (C)p0.bg[K];
+#pragma unroll
+      for (int l = 0; l < NDIM; l++) bgl[l] = (C)p0.bg[l];
+    }
+    const C inv12 = idx[j] / (C)12;
+    C Fm[NE], D1[NE], E1[NE], vm[NDIM], dv1[NDIM];
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      load_pt<NDIR, S, C>(W, B, fs, ic - sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, Fm, vm);
+    }
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      C F[NE], v[NDIM];
+      load_pt<NDIR, S, C>(W, B, fs, ic + sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, F, v);
+#pragma unroll
+      for (int n = 0; n < NE; n++) D1[n] = two_diff(F[n], Fm[n], E1[n]);
+#pragma unroll
+      for (int l = 0; l < NDIM; l++) dv1[l] = v[l] - vm[l];
+    }
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      load_pt<NDIR, S, C>(W, B, fs, ic - 2 * sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, Fm, vm);
+    }
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      C F[NE], v[NDIM];
+      load_pt<NDIR, S, C>(W, B, fs, ic + 2 * sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, F, v);
+      // -gradient4 = ((F2-F-2) - 8 (F1-F-1)) / (12 dx), with the rounding
+      // errors of the two differences added back (zero in FP64)
+#pragma unroll
+      for (int n = 0; n < NE; n++) {
+        C e2;
+        const C D2 = two_diff(F[n], Fm[n], e2);
+        R[j][n] = ((D2 - (C)8 * D1[n]) + (e2 - (C)8 * E1[n])) * inv12;
+      }
+      C re = 0;
+#pragma unroll
+      for (int l = 0; l < NDIM; l++) {
+        const C Dv = -((v[l] - vm[l]) - (C)8 * dv1[l]) * inv12;
+        if (l == K) re -= Dv * pTb0;
+        re += bgk * bgl[l] * Dv;
+      }
+      R[j][1 + NDIR] += re;
+#pragma unroll
+      for (int n = 0; n < NE; n++) R[j][n] = fluxc[j] ? R[j][n] : (C)0;
+    }
+    SCHED_FENCE();
+  }
+
+  // ---- hyperdiffusion along K ----
+  if (a.visc_on) {
+    const C ctk = (C)red_get(a.ctot_src, K);
+    constexpr int NQ = CK + 6;  // stencil points at offsets -3..CK+2
+    int off[NQ];
+    double inv7[NQ];
+#pragma unroll
+    for (int s = 0; s < NQ; s++) {
+      int j = c0 + s - 3;
+      if (j < 0) j = MloK + 3;       // mirror layer below the grid
+      if (j > GK - 1) j = MhiK - 3;  // and above it
+      off[s] = (j - c0) * (int)sk;
+      const long long i = ic0 + off[s];
+      inv7[s] = recip<C>((double)W[i] + B[fs + i]);
+    }
+    // rho_t of the cells and their direct neighbours (offsets -1..CK)
+    C rtw[CK + 2];
+    int offn[CK + 2];
+#pragma unroll
+    for (int s = 0; s < CK + 2; s++) {
+      int j = c0 + s - 1;
+      j = j > GK - 1 ? GK - 1 : j;
+      offn[s] = (j - c0) * (int)sk;
+      const long long i = ic0 + offn[s];
+      rtw[s] = (C)((double)W[i] + B[fs + i]);
+    }
+    // p = 0: rho' ; p = 1: thermal energy ; p = 2+l: v_l ; then b_l (l != K)
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      C q[NQ];
+      int lb = -1;
+      if (p >= 2 + NDIM) {
+        int n = p - 2 - NDIM;
+        lb = (n < K) ? n : n + 1;
+      }
+#pragma unroll
+      for (int s = 0; s < NQ; s++) {
+        const long long i = ic0 + off[s];
+        if (p == 0) {
+          q[s] = (C)W[i];
+        } else if (p == 1) {
+          // thermal energy e - (b^2 + m^2/rho_t)/2, recovered in FP64
+          double sb = 0.0, sm = 0.0;
+#pragma unroll
+          for (int cd = 0; cd < NDIR; cd++) {
+            const double mc = (double)W[(1 + cd) * fs + i];
+            const double bc = (double)W[(2 + NDIR + cd) * fs + i];
+            sb += bc * bc;
+            sm += mc * mc;
+          }
+          q[s] = (C)((double)W[(1 + NDIR) * fs + i] - 0.5 * (sb + sm * inv7[s]));
+        } else if (p < 2 + NDIM) {
+          q[s] = (C)W[(p - 1) * fs + i] * (C)inv7[s];
+        } else {
+          q[s] = (C)W[(2 + NDIR + lb) * fs + i];
+        }
+      }
+      const C ch = (p >= 2 && p < 2 + NDIM) ? (C)0.4 : (C)0.04;
+      const C Kc = ctk * ch;
+      // face f (between offsets f-1 and f) from the window q[f..f+5]
+      C ratio[CK + 1];
+#pragma unroll
+      for (int f = 0; f < CK + 1; f++) ratio[f] = face_ratio<C>(q + f);
+#pragma unroll
+      for (int j = 0; j < CK; j++) {
+        const C nuR = Kc * ratio[j + 1] * dxk[j];
+        const C nuL = Kc * ratio[j] * dxk[j];
+        numax = valid[j] ? fmax(numax, fmax(nuR, nuL)) : numax;
+        const C qm = q[j + 2], q0 = q[j + 3], qp = q[j + 4];
+        const C gL = z1[j] ? (C)0 : (q0 - qm) * idx[j];
+        const C gR = z1[j] ? (C)0 : (qp - q0) * idx[j];
+        const C tL = nuL * gL, tR = nuR * gR;
+        const long long icj = ic0 + offn[j + 1];
+        if (p == 0) {
+          R[j][0] += (tR - tL) * idx[j];
+        } else if (p == 1) {
+          R[j][1 + NDIR] += (tR - tL) * idx[j];
+        } else if (p < 2 + NDIM) {
+          const int l = p - 2;
+          const C rhoL = (rtw[j + 1] + rtw[j]) * (C)0.5, rhoR = (rtw[j + 2] + rtw[j + 1]) * (C)0.5;
+          const S* __restrict__ Wm = W + (1 + l) * fs + ic0;
+          const C m0 = (C)Wm[offn[j + 1]];
+          const C mL = (m0 + (C)Wm[offn[j]]) * (C)0.5;
+          const C mR = ((C)Wm[offn[j + 2]] + m0) * (C)0.5;
+          const C mult = (l == K) ? (C)1 : (C)0.5;  // the i==k branch runs twice for l==k
+          R[j][1 + l] += mult * (rhoR * tR - rhoL * tL) * idx[j];
+          R[j][1 + NDIR] += mult * (mR * tR - mL * tL) * idx[j];
+          if (l != K) {
+            const C gC = z1[j] ? (C)0 : (qp - qm) * idx[j] * (C)0.5;
+            ((S*)a.T)[(long long)cross_index(NDIM, K, l) * fs + icj] =
+                (S)(gC * (nuL + nuR) * (C)0.25);
+          }
+        } else {
+          const C BL = (q0 + qm) * (C)0.5, BR = (qp + q0) * (C)0.5;
+          R[j][2 + NDIR + lb] += (tR - tL) * idx[j];
+          R[j][1 + NDIR] += (tR * BR - tL * BL) * idx[j];
+          const C gC = z1[j] ? (C)0 : (qp - qm) * idx[j] * (C)0.5;
+          ((S*)a.T)[(long long)(cross_index(NDIM, K, lb) + 1) * fs + icj] =
+              (S)(gC * (nuL + nuR) * (C)0.5);
+        }
+      }
+      SCHED_FENCE();
+    }
+  }
+  // ---- accumulate ----
+  // Stores are unconditional (branches here cost registers): a cell past
+  // the mesh end writes R and T of a ghost cell (index clamped to the
+  // grid), which nothing reads.
+  S* __restrict__ Racc = (S*)a.R;
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    const int cj = (c0 + j > GK - 1) ? GK - 1 : c0 + j;
+    const long long icj = ic0 + (long long)(cj - c0) * sk;
+    if (K == 0) {
+#pragma unroll
+      for (int n = 0; n < NE; n++) Racc[n * fs + icj] = (S)R[j][n];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NE; n++) Racc[n * fs + icj] = (S)((C)Racc[n * fs + icj] + R[j][n]);
+    }
+  }
+}
+
+template <int NDIM, int NDIR, int K, typename S, typename C>
+__global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
+  constexpr int CK = LineGroup<K>::CK;
   const GridDesc& g = a.g;
   const long long t = xcd_block(a.xcd) * blockDim.x + threadIdx.x;
   int lo[3], ext[3];
@@ -255,183 +527,26 @@ __global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
     lo[d] = g.Mlo[d];
     ext[d] = g.Mhi[d] - g.Mlo[d] + 1;
   }
+  // groups of CK cells along K
+  ext[K] = (ext[K] + CK - 1) / CK;
   const bool active = t < (long long)ext[0] * ext[1] * ext[2];
-  double numax = 0.0;
+  C numax = 0;
   if (active) {
     int c[3];
-    decode_box(t, lo, ext, c);
-    const long long fs = g.fs;
-    const double* __restrict__ W = a.wct;
-    const double* __restrict__ B = a.bg;
-    const double gamma = a.gamma;
-    const long long ic = lin(g, c[0], c[1], c[2]);
-    const long long sk = g.st[K];
-    const double dxk = a.dx[K][c[K]];
-    const double idx = 1.0 / dxk;
-    double R[NE];
-#pragma unroll
-    for (int v = 0; v < NE; v++) R[v] = 0.0;
-
-    // ---- flux divergence along K (zero on two layers at each side) ----
-    if (!((c[K] <= g.Mlo[K] + 1) || (c[K] >= g.Mhi[K] - 1))) {
-      double bgk, bgl[NDIM], pTb0;
-      {
-        Pt<NDIR> p0;
-        load_pt<NDIR>(W, B, fs, ic, p0);
-        pTb0 = ptotal_bg<NDIR>(p0, gamma);
-        bgk = p0.bg[K];
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) bgl[l] = p0.bg[l];
-      }
-      const double inv12 = idx / 12.0;
-      double Fm[NE], D1[NE], vm[NDIM], dv1[NDIM];
-      SCHED_FENCE();
-      {
-        Pt<NDIR> p;
-        load_pt<NDIR>(W, B, fs, ic - sk, p);
-        flux_pt<NDIR>(p, K, gamma, Fm);
-        const double inv = 1.0 / (p.rho + p.rhob);
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) vm[l] = p.m[l] * inv;
-      }
-      SCHED_FENCE();
-      {
-        Pt<NDIR> p;
-        double F[NE];
-        load_pt<NDIR>(W, B, fs, ic + sk, p);
-        flux_pt<NDIR>(p, K, gamma, F);
-        const double inv = 1.0 / (p.rho + p.rhob);
-#pragma unroll
-        for (int v = 0; v < NE; v++) D1[v] = F[v] - Fm[v];
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) dv1[l] = p.m[l] * inv - vm[l];
-      }
-      SCHED_FENCE();
-      {
-        Pt<NDIR> p;
-        load_pt<NDIR>(W, B, fs, ic - 2 * sk, p);
-        flux_pt<NDIR>(p, K, gamma, Fm);
-        const double inv = 1.0 / (p.rho + p.rhob);
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) vm[l] = p.m[l] * inv;
-      }
-      SCHED_FENCE();
-      {
-        Pt<NDIR> p;
-        double F[NE];
-        load_pt<NDIR>(W, B, fs, ic + 2 * sk, p);
-        flux_pt<NDIR>(p, K, gamma, F);
-        const double inv = 1.0 / (p.rho + p.rhob);
-        // -gradient4 = ((F2-F-2) - 8 (F1-F-1)) / (12 dx)
-#pragma unroll
-        for (int v = 0; v < NE; v++) R[v] += ((F[v] - Fm[v]) - 8.0 * D1[v]) * inv12;
-        double re = 0.0;
-#pragma unroll
-        for (int l = 0; l < NDIM; l++) {
-          const double Dv = -((p.m[l] * inv - vm[l]) - 8.0 * dv1[l]) * inv12;
-          if (l == K) re -= Dv * pTb0;
-          re += bgk * bgl[l] * Dv;
-        }
-        R[1 + NDIR] += re;
-      }
+    {
+      long long u = t;
+      c[0] = (int)(u % ext[0]);
+      u /= ext[0];
+      c[1] = (int)(u % ext[1]);
+      u /= ext[1];
+      c[2] = (int)u;
     }
-    SCHED_FENCE();
-
-    // ---- hyperdiffusion along K ----
-    if (a.visc_on) {
-      const double ctk = red_get(a.ctot_src, K);
-      // rho_t and 1/rho_t at offsets -3..3 (mirror beyond the grid)
-      long long id7[7];
-      double inv7[7];
 #pragma unroll
-      for (int s = 0; s < 7; s++) {
-        int j = c[K] + s - 3;
-        if (j < 0) j = g.Mlo[K] + 3;
-        if (j > g.G[K] - 1) j = g.Mhi[K] - 3;
-        id7[s] = ic + (long long)(j - c[K]) * sk;
-        inv7[s] = 1.0 / (W[id7[s]] + B[fs + id7[s]]);
-      }
-      const bool z1 = (c[K] == g.Mlo[K] && !g.per_lo[K]) || (c[K] == g.Mhi[K] && !g.per_hi[K]);
-      const double rtm = W[ic - sk] + B[fs + ic - sk];
-      const double rt0 = W[ic] + B[fs + ic];
-      const double rtp = W[ic + sk] + B[fs + ic + sk];
-      // p = 0: rho' ; p = 1: thermal energy ; p = 2+l: v_l ; then b_l (l != K)
-#pragma unroll
-      for (int p = 0; p < NP; p++) {
-        double q[7];
-        int lb = -1;
-        if (p >= 2 + NDIM) {
-          int n = p - 2 - NDIM;
-          lb = (n < K) ? n : n + 1;
-        }
-#pragma unroll
-        for (int s = 0; s < 7; s++) {
-          const long long i = id7[s];
-          if (p == 0) {
-            q[s] = W[i];
-          } else if (p == 1) {
-            double sb = 0.0, sm = 0.0;
-#pragma unroll
-            for (int cd = 0; cd < NDIR; cd++) {
-              const double mc = W[(1 + cd) * fs + i];
-              const double bc = W[(2 + NDIR + cd) * fs + i];
-              sb += bc * bc;
-              sm += mc * mc;
-            }
-            q[s] = W[(1 + NDIR) * fs + i] - 0.5 * (sb + sm * inv7[s]);
-          } else if (p < 2 + NDIM) {
-            q[s] = W[(p - 1) * fs + i] * inv7[s];
-          } else {
-            q[s] = W[(2 + NDIR + lb) * fs + i];
-          }
-        }
-        const double ch = (p >= 2 && p < 2 + NDIM) ? 0.4 : 0.04;
-        const double Kc = ctk * ch;
-        const double nuR = Kc * face_ratio(q + 1) * dxk;
-        const double nuL = Kc * face_ratio(q) * dxk;
-        numax = fmax(numax, fmax(nuR, nuL));
-        const double gL = z1 ? 0.0 : (q[3] - q[2]) * idx;
-        const double gR = z1 ? 0.0 : (q[4] - q[3]) * idx;
-        const double tL = nuL * gL, tR = nuR * gR;
-        if (p == 0) {
-          R[0] += (tR - tL) * idx;
-        } else if (p == 1) {
-          R[1 + NDIR] += (tR - tL) * idx;
-        } else if (p < 2 + NDIM) {
-          const int l = p - 2;
-          const double rhoL = (rt0 + rtm) * 0.5, rhoR = (rtp + rt0) * 0.5;
-          const double m0 = W[(1 + l) * fs + ic];
-          const double mL = (m0 + W[(1 + l) * fs + ic - sk]) * 0.5;
-          const double mR = (W[(1 + l) * fs + ic + sk] + m0) * 0.5;
-          const double mult = (l == K) ? 1.0 : 0.5;  // the i==k branch runs twice for l==k
-          R[1 + l] += mult * (rhoR * tR - rhoL * tL) * idx;
-          R[1 + NDIR] += mult * (mR * tR - mL * tL) * idx;
-          if (l != K) {
-            const double gC = z1 ? 0.0 : (q[4] - q[2]) * idx * 0.5;
-            a.T[(long long)cross_index(NDIM, K, l) * fs + ic] = gC * (nuL + nuR) * 0.25;
-          }
-        } else {
-          const double BL = (q[3] + q[2]) * 0.5, BR = (q[4] + q[3]) * 0.5;
-          R[2 + NDIR + lb] += (tR - tL) * idx;
-          R[1 + NDIR] += (tR * BR - tL * BL) * idx;
-          const double gC = z1 ? 0.0 : (q[4] - q[2]) * idx * 0.5;
-          a.T[(long long)(cross_index(NDIM, K, lb) + 1) * fs + ic] = gC * (nuL + nuR) * 0.5;
-        }
-        SCHED_FENCE();
-      }
-    }
-    // ---- accumulate ----
-    double* __restrict__ Racc = a.R;
-    if (K == 0) {
-#pragma unroll
-      for (int v = 0; v < NE; v++) Racc[v * fs + ic] = R[v];
-    } else {
-#pragma unroll
-      for (int v = 0; v < NE; v++) Racc[v * fs + ic] += R[v];
-    }
+    for (int d = 0; d < 3; d++) c[d] = lo[d] + ((d == K) ? c[d] * CK : c[d]);
+    axis_group<NDIM, NDIR, K, CK, S, C>(a, g, c, numax);
   }
   if (a.visc_on) {
-    double r2[1] = {numax};
+    double r2[1] = {(double)numax};
     block_max_atomic<1>(r2, a.red_visc);
   }
 }
@@ -444,10 +559,102 @@ __global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
 //   b_k -= d_l(Tb_{k,l}),        e -= d_l(b_k Tb_{k,l})
 // with central differences along l, zero at a non-periodic edge of l, and
 // T = 0 outside the mesh (nu is never set there, src/vacusr.viscosity.t:116-129).
+// The register update runs in FP64 and is rounded once to S.
 
-template <int NDIM, int NDIR>
-__global__ void __launch_bounds__(256) k_final(FinalArgs a) {
+// One mesh cell c of the grid g; the cmax values of the result go to redv.
+template <int NDIM, int NDIR, typename S, typename C>
+__device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g, const int (&c)[3],
+                                           double (&redv)[2 * NDIM]) {
   constexpr int NE = 2 + 2 * NDIR;
+  const long long fs = g.fs;
+  const S* __restrict__ W = (const S*)a.wct;
+  const S* __restrict__ RR = (const S*)a.R;
+  const S* __restrict__ TT = (const S*)a.T;
+  const double* __restrict__ B = a.bg;
+  const long long ic = lin(g, c[0], c[1], c[2]);
+  C R[NE];
+#pragma unroll
+  for (int n = 0; n < NE; n++) R[n] = (C)RR[n * fs + ic];
+  // gravity on the perturbed density (addsource_grav, src/vacusr.gravity.t:46-78)
+  {
+    const C rho = (C)W[ic];
+    const C inv = (C)recip<C>((double)W[ic] + B[fs + ic]);
+    C rg = 0;
+#pragma unroll
+    for (int d = 0; d < NDIM; d++) {
+      if (a.grav_on[d]) {
+        const C gd = (C)a.grav[d];
+        R[1 + d] += gd * rho;
+        rg += gd * rho * (C)W[(1 + d) * fs + ic] * inv;
+      }
+    }
+    R[1 + NDIR] += rg;
+  }
+  if (a.visc_on && NDIM > 1) {
+#pragma unroll
+    for (int l = 0; l < NDIM; l++) {
+      if ((c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l])) continue;
+      const long long sl = g.st[l];
+      const C h = (C)(0.5 / a.dx[l][c[l]]);
+      const bool inP = c[l] + 1 <= g.Mhi[l], inM = c[l] - 1 >= g.Mlo[l];
+      const long long ip = ic + sl, im = ic - sl;
+      const C rtp = (C)((double)W[ip] + B[fs + ip]), rtm = (C)((double)W[im] + B[fs + im]);
+#pragma unroll
+      for (int k = 0; k < NDIM; k++) {
+        if (k == l) continue;
+        const long long xi = (long long)cross_index(NDIM, k, l) * fs;
+        const C Tmp = inP ? (C)TT[xi + ip] : (C)0, Tmm = inM ? (C)TT[xi + im] : (C)0;
+        const C Tbp = inP ? (C)TT[xi + fs + ip] : (C)0, Tbm = inM ? (C)TT[xi + fs + im] : (C)0;
+        const C mkp = (C)W[(1 + k) * fs + ip], mkm = (C)W[(1 + k) * fs + im];
+        const C bkp = (C)W[(2 + NDIR + k) * fs + ip], bkm = (C)W[(2 + NDIR + k) * fs + im];
+        R[1 + k] += (rtp * Tmp - rtm * Tmm) * h;
+        R[1 + NDIR] += (mkp * Tmp - mkm * Tmm) * h;
+        R[2 + NDIR + k] -= (Tbp - Tbm) * h;
+        R[1 + NDIR] -= (Tbp * bkp - Tbm * bkm) * h;
+      }
+    }
+  }
+  // register update and cmax of the (stored) result
+  Pt<NDIR, double> po;
+  {
+    S* __restrict__ out = (S*)a.out;
+    const S* __restrict__ base = (const S*)a.base;
+    double vals[NE];
+#pragma unroll
+    for (int n = 0; n < NE; n++) {
+      const S o = (S)((double)base[n * fs + ic] + a.qa * (double)R[n]);
+      out[n * fs + ic] = o;
+      vals[n] = (double)o;
+    }
+    if (a.acc_out) {
+      S* __restrict__ ao = (S*)a.acc_out;
+      const S* __restrict__ ai = (const S*)a.acc_in;
+#pragma unroll
+      for (int n = 0; n < NE; n++)
+        ao[n * fs + ic] = (S)((double)ai[n * fs + ic] + a.qb * (double)R[n]);
+    }
+    po.rho = vals[0];
+#pragma unroll
+    for (int cd = 0; cd < NDIR; cd++) {
+      po.m[cd] = vals[1 + cd];
+      po.b[cd] = vals[2 + NDIR + cd];
+      po.bg[cd] = B[(2 + cd) * fs + ic];
+    }
+    po.e = vals[1 + NDIR];
+    po.eb = B[ic];
+    po.rhob = B[fs + ic];
+  }
+  double cm[NDIM];
+  cmax_all<NDIM, NDIR>(po, a.gamma, cm);
+#pragma unroll
+  for (int d = 0; d < NDIM; d++) {
+    redv[d] = cm[d];
+    redv[NDIM + d] = cm[d] / a.dx[d][c[d]];
+  }
+}
+
+template <int NDIM, int NDIR, typename S, typename C>
+__global__ void __launch_bounds__(256) k_final(FinalArgs a) {
   const GridDesc& g = a.g;
   const long long t = xcd_block(a.xcd) * blockDim.x + threadIdx.x;
   int lo[3], ext[3];
@@ -463,81 +670,7 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
   if (active) {
     int c[3];
     decode_box(t, lo, ext, c);
-    const long long fs = g.fs;
-    const double* __restrict__ W = a.wct;
-    const double* __restrict__ B = a.bg;
-    const long long ic = lin(g, c[0], c[1], c[2]);
-    double R[NE];
-#pragma unroll
-    for (int v = 0; v < NE; v++) R[v] = a.R[v * fs + ic];
-    // gravity on the perturbed density (addsource_grav, src/vacusr.gravity.t:46-78)
-    {
-      const double rho = W[ic];
-      const double inv = 1.0 / (rho + B[fs + ic]);
-      double rg = 0.0;
-#pragma unroll
-      for (int d = 0; d < NDIM; d++) {
-        if (a.grav_on[d]) {
-          R[1 + d] += a.grav[d] * rho;
-          rg += a.grav[d] * rho * W[(1 + d) * fs + ic] * inv;
-        }
-      }
-      R[1 + NDIR] += rg;
-    }
-    if (a.visc_on && NDIM > 1) {
-#pragma unroll
-      for (int l = 0; l < NDIM; l++) {
-        if ((c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l])) continue;
-        const long long sl = g.st[l];
-        const double h = 0.5 / a.dx[l][c[l]];
-        const bool inP = c[l] + 1 <= g.Mhi[l], inM = c[l] - 1 >= g.Mlo[l];
-        const long long ip = ic + sl, im = ic - sl;
-        const double rtp = W[ip] + B[fs + ip], rtm = W[im] + B[fs + im];
-#pragma unroll
-        for (int k = 0; k < NDIM; k++) {
-          if (k == l) continue;
-          const long long xi = (long long)cross_index(NDIM, k, l) * fs;
-          const double Tmp = inP ? a.T[xi + ip] : 0.0, Tmm = inM ? a.T[xi + im] : 0.0;
-          const double Tbp = inP ? a.T[xi + fs + ip] : 0.0, Tbm = inM ? a.T[xi + fs + im] : 0.0;
-          const double mkp = W[(1 + k) * fs + ip], mkm = W[(1 + k) * fs + im];
-          const double bkp = W[(2 + NDIR + k) * fs + ip], bkm = W[(2 + NDIR + k) * fs + im];
-          R[1 + k] += (rtp * Tmp - rtm * Tmm) * h;
-          R[1 + NDIR] += (mkp * Tmp - mkm * Tmm) * h;
-          R[2 + NDIR + k] -= (Tbp - Tbm) * h;
-          R[1 + NDIR] -= (Tbp * bkp - Tbm * bkm) * h;
-        }
-      }
-    }
-    // register update and cmax of the result
-    Pt<NDIR> po;
-    {
-      double vals[NE];
-#pragma unroll
-      for (int v = 0; v < NE; v++) {
-        vals[v] = a.base[v * fs + ic] + a.qa * R[v];
-        a.out[v * fs + ic] = vals[v];
-      }
-      if (a.acc_out) {
-#pragma unroll
-        for (int v = 0; v < NE; v++) a.acc_out[v * fs + ic] = a.acc_in[v * fs + ic] + a.qb * R[v];
-      }
-      po.rho = vals[0];
-#pragma unroll
-      for (int cd = 0; cd < NDIR; cd++) {
-        po.m[cd] = vals[1 + cd];
-        po.b[cd] = vals[2 + NDIR + cd];
-        po.bg[cd] = B[(2 + cd) * fs + ic];
-      }
-      po.e = vals[1 + NDIR];
-      po.eb = B[ic];
-      po.rhob = B[fs + ic];
-    }
-#pragma unroll
-    for (int d = 0; d < NDIM; d++) {
-      const double cm = cmax_pt<NDIR>(po, d, a.gamma);
-      redv[d] = cm;
-      redv[NDIM + d] = cm / a.dx[d][c[d]];
-    }
+    final_cell<NDIM, NDIR, S, C>(a, g, c, redv);
   }
   block_max_atomic<2 * NDIM>(redv, a.red_out);
 }
@@ -545,7 +678,7 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
 // ---------------------------------------------------------------------
 // kernel 3: cmax reductions of a register (getdt_courant / setnu c_tot)
 
-template <int NDIM, int NDIR>
+template <int NDIM, int NDIR, typename S>
 __global__ void __launch_bounds__(256) k_cmax(CmaxArgs a) {
   const GridDesc& g = a.g;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -561,13 +694,14 @@ __global__ void __launch_bounds__(256) k_cmax(CmaxArgs a) {
   if (t < (long long)ext[0] * ext[1] * ext[2]) {
     int c[3];
     decode_box(t, lo, ext, c);
-    Pt<NDIR> p;
-    load_pt<NDIR>(a.w, a.bg, g.fs, lin(g, c[0], c[1], c[2]), p);
+    Pt<NDIR, double> p;
+    load_pt<NDIR, S, double>((const S*)a.w, a.bg, g.fs, lin(g, c[0], c[1], c[2]), p);
+    double cm[NDIM];
+    cmax_all<NDIM, NDIR>(p, a.gamma, cm);
 #pragma unroll
     for (int d = 0; d < NDIM; d++) {
-      const double cm = cmax_pt<NDIR>(p, d, a.gamma);
-      r[d] = cm;
-      r[NDIM + d] = cm / a.dx[d][c[d]];
+      r[d] = cm[d];
+      r[NDIM + d] = cm[d] / a.dx[d][c[d]];
     }
   }
   block_max_atomic<2 * NDIM>(r, a.red_out);
@@ -582,10 +716,13 @@ __global__ void __launch_bounds__(256) k_cmax(CmaxArgs a) {
 // the stored value of a 'fixed' region), which is what every thread does
 // here; no thread reads a ghost cell, so the kernel is race-free.
 
-__device__ __forceinline__ double ghost_value(const GhostArgs& a, int var, int c0, int c1, int c2) {
+template <typename S>
+__device__ __forceinline__ S ghost_value(const GhostArgs& a, int var, int c0, int c1, int c2) {
   const GridDesc& g = a.g;
+  const S* f = (const S*)a.f;
+  const S* snap = (const S*)a.snap;
   int c[3] = {c0, c1, c2};
-  double sign = 1.0;
+  bool neg = false;
   for (int iter = 0; iter < 8; iter++) {
     int d = -1;
 #pragma unroll
@@ -593,26 +730,27 @@ __device__ __forceinline__ double ghost_value(const GhostArgs& a, int var, int c
       if (d < 0 && dd < g.ndim && (c[dd] < g.Mlo[dd] || c[dd] > g.Mhi[dd])) d = dd;
     }
     const long long i = lin(g, c[0], c[1], c[2]);
-    if (d < 0) return sign * a.f[(long long)var * g.fs + i];
+    if (d < 0) return neg ? -f[(long long)var * g.fs + i] : f[(long long)var * g.fs + i];
     const int side = (c[d] > g.Mhi[d]) ? 1 : 0;
     const int code = a.bc[var * 6 + 2 * d + side];
     const int N = g.Mhi[d] - g.Mlo[d] + 1;
     if (code == BC_PERIODIC) {
       c[d] += side ? -N : N;
-    } else if (code == BC_FIXED && a.snap) {
-      return sign * a.snap[(long long)var * g.fs + i];
+    } else if (code == BC_FIXED && snap) {
+      return neg ? -snap[(long long)var * g.fs + i] : snap[(long long)var * g.fs + i];
     } else if (code == BC_FIXED && a.keep_fixed) {
-      return sign * a.f[(long long)var * g.fs + i];
+      return neg ? -f[(long long)var * g.fs + i] : f[(long long)var * g.fs + i];
     } else if (code == BC_CONT || code == BC_FIXED) {
       c[d] = side ? g.Mhi[d] : g.Mlo[d];
     } else {  // symm, symm0, asymm: reflection about the boundary face
       c[d] = side ? (2 * g.Mhi[d] + 1 - c[d]) : (2 * g.Mlo[d] - 1 - c[d]);
-      if (code == BC_ASYMM) sign = -sign;
+      if (code == BC_ASYMM) neg = !neg;
     }
   }
-  return 0.0;
+  return (S)0;
 }
 
+template <typename S>
 __global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
   const GridDesc& g = a.g;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -654,9 +792,10 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
     return;
   }
   const long long io = lin(g, c[0], c[1], c[2]);
+  S* f = (S*)a.f;
   for (int v = 0; v < a.nvar; v++) {
-    const double val = ghost_value(a, v, c[0], c[1], c[2]);
-    a.f[(long long)v * g.fs + io] = val;
+    const S val = ghost_value<S>(a, v, c[0], c[1], c[2]);
+    f[(long long)v * g.fs + io] = val;
   }
 }
 
@@ -668,24 +807,30 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
-// w3 <- (w1 + 2 w2 + w3 - 4 w)/3 style combination on whole registers:
-// out = c0*a0 + c1*a1 + c2*a2 + c3*a3 (null operands skipped)
-__global__ void k_axpy4(long long n, double* __restrict__ out, const double* a0, double c0,
-                        const double* a1, double c1, const double* a2, double c2, const double* a3,
-                        double c3) {
+// out = c0*a0 + c1*a1 + c2*a2 + c3*a3 in FP64 (null operands skipped)
+template <typename S>
+__global__ void k_axpy4(long long n, S* __restrict__ out, const S* a0, double c0, const S* a1,
+                        double c1, const S* a2, double c2, const S* a3, double c3) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s = c0 * a0[i];
-  if (a1) s += c1 * a1[i];
-  if (a2) s += c2 * a2[i];
-  if (a3) s += c3 * a3[i];
-  out[i] = s;
+  double s = c0 * (double)a0[i];
+  if (a1) s += c1 * (double)a1[i];
+  if (a2) s += c2 * (double)a2[i];
+  if (a3) s += c3 * (double)a3[i];
+  out[i] = (S)s;
+}
+
+// type conversion of n elements (FP64 staging <-> FP32 registers)
+template <typename D, typename S>
+__global__ void k_convert(long long n, D* __restrict__ out, const S* __restrict__ in) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (D)in[i];
 }
 
 // ---------------------------------------------------------------------
 // launchers
 
-template <int NDIM, int NDIR>
+template <int NDIM, int NDIR, typename S, typename C>
 static void launch_nd(int which, int axis, const void* args, hipStream_t st, long long nthreads) {
   const int bs = 256;
   unsigned nb = (unsigned)((nthreads + bs - 1) / bs);
@@ -693,26 +838,39 @@ static void launch_nd(int which, int axis, const void* args, hipStream_t st, lon
   if (nb == 0) return;
   if (which == 0) {
     const AxisArgs& A = *(const AxisArgs*)args;
-    if (axis == 0) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0>), dim3(nb), dim3(bs), 0, st, A);
+    if (axis == 0) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0, S, C>), dim3(nb), dim3(bs), 0, st, A);
     if constexpr (NDIM > 1)
-      if (axis == 1) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 1>), dim3(nb), dim3(bs), 0, st, A);
+      if (axis == 1)
+        hipLaunchKernelGGL((k_axis<NDIM, NDIR, 1, S, C>), dim3(nb), dim3(bs), 0, st, A);
     if constexpr (NDIM > 2)
-      if (axis == 2) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2>), dim3(nb), dim3(bs), 0, st, A);
+      if (axis == 2)
+        hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2, S, C>), dim3(nb), dim3(bs), 0, st, A);
   } else if (which == 1) {
-    hipLaunchKernelGGL((k_final<NDIM, NDIR>), dim3(nb), dim3(bs), 0, st, *(const FinalArgs*)args);
+    hipLaunchKernelGGL((k_final<NDIM, NDIR, S, C>), dim3(nb), dim3(bs), 0, st,
+                       *(const FinalArgs*)args);
   } else {
-    hipLaunchKernelGGL((k_cmax<NDIM, NDIR>), dim3(nb), dim3(bs), 0, st, *(const CmaxArgs*)args);
+    hipLaunchKernelGGL((k_cmax<NDIM, NDIR, S>), dim3(nb), dim3(bs), 0, st,
+                       *(const CmaxArgs*)args);
   }
 }
 
-static void dispatch(int ndim, int ndir, int which, int axis, const void* args, hipStream_t st,
-                     long long n) {
-  if (ndim == 1 && ndir == 1) launch_nd<1, 1>(which, axis, args, st, n);
-  else if (ndim == 1 && ndir == 2) launch_nd<1, 2>(which, axis, args, st, n);
-  else if (ndim == 1 && ndir == 3) launch_nd<1, 3>(which, axis, args, st, n);
-  else if (ndim == 2 && ndir == 2) launch_nd<2, 2>(which, axis, args, st, n);
-  else if (ndim == 2 && ndir == 3) launch_nd<2, 3>(which, axis, args, st, n);
-  else if (ndim == 3 && ndir == 3) launch_nd<3, 3>(which, axis, args, st, n);
+template <typename S, typename C>
+static void dispatch_t(int ndim, int ndir, int which, int axis, const void* args, hipStream_t st,
+                       long long n) {
+  if (ndim == 1 && ndir == 1) launch_nd<1, 1, S, C>(which, axis, args, st, n);
+  else if (ndim == 1 && ndir == 2) launch_nd<1, 2, S, C>(which, axis, args, st, n);
+  else if (ndim == 1 && ndir == 3) launch_nd<1, 3, S, C>(which, axis, args, st, n);
+  else if (ndim == 2 && ndir == 2) launch_nd<2, 2, S, C>(which, axis, args, st, n);
+  else if (ndim == 2 && ndir == 3) launch_nd<2, 3, S, C>(which, axis, args, st, n);
+  else if (ndim == 3 && ndir == 3) launch_nd<3, 3, S, C>(which, axis, args, st, n);
+}
+
+static void dispatch(int prec, int ndim, int ndir, int which, int axis, const void* args,
+                     hipStream_t st, long long n) {
+  if (prec == PREC_MIXED)
+    dispatch_t<float, float>(ndim, ndir, which, axis, args, st, n);
+  else
+    dispatch_t<double, double>(ndim, ndir, which, axis, args, st, n);
 }
 
 static long long mesh_cells(const GridDesc& g) {
@@ -722,15 +880,25 @@ static long long mesh_cells(const GridDesc& g) {
 }
 
 void launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
-  dispatch(a.g.ndim, a.g.ndir, 0, axis, &a, st, mesh_cells(a.g));
+  const GridDesc& g = a.g;
+  long long n = 1;
+  for (int d = 0; d < 3; d++) {
+    long long e = g.Mhi[d] - g.Mlo[d] + 1;
+    if (d == axis) {
+      const int ck = (axis == 0) ? SAC_CK_X : SAC_CK_YZ;
+      e = (e + ck - 1) / ck;
+    }
+    n *= e;
+  }
+  dispatch(a.prec, g.ndim, g.ndir, 0, axis, &a, st, n);
 }
 
 void launch_final(const FinalArgs& a, hipStream_t st) {
-  dispatch(a.g.ndim, a.g.ndir, 1, 0, &a, st, mesh_cells(a.g));
+  dispatch(a.prec, a.g.ndim, a.g.ndir, 1, 0, &a, st, mesh_cells(a.g));
 }
 
 void launch_cmax(const CmaxArgs& a, hipStream_t st) {
-  dispatch(a.g.ndim, a.g.ndir, 2, 0, &a, st, mesh_cells(a.g));
+  dispatch(a.prec, a.g.ndim, a.g.ndir, 2, 0, &a, st, mesh_cells(a.g));
 }
 
 void launch_ghost(const GhostArgs& a, hipStream_t st) {
@@ -739,19 +907,41 @@ void launch_ghost(const GhostArgs& a, hipStream_t st) {
   long long n = ncell - mesh_cells(g);
   if (n <= 0) return;
   const int bs = 256;
-  hipLaunchKernelGGL(k_ghost, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st, a);
+  const dim3 grid((unsigned)((n + bs - 1) / bs));
+  if (a.prec == PREC_MIXED)
+    hipLaunchKernelGGL(k_ghost<float>, grid, dim3(bs), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_ghost<double>, grid, dim3(bs), 0, st, a);
 }
 
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u64, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
 }
 
-void launch_axpy4(long long n, double* out, const double* a0, double c0, const double* a1,
-                  double c1, const double* a2, double c2, const double* a3, double c3,
+void launch_axpy4(int prec, long long n, void* out, const void* a0, double c0, const void* a1,
+                  double c1, const void* a2, double c2, const void* a3, double c3,
                   hipStream_t st) {
   const int bs = 256;
-  hipLaunchKernelGGL(k_axpy4, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st, n, out, a0,
-                     c0, a1, c1, a2, c2, a3, c3);
+  const dim3 grid((unsigned)((n + bs - 1) / bs));
+  if (prec == PREC_MIXED)
+    hipLaunchKernelGGL(k_axpy4<float>, grid, dim3(bs), 0, st, n, (float*)out, (const float*)a0,
+                       c0, (const float*)a1, c1, (const float*)a2, c2, (const float*)a3, c3);
+  else
+    hipLaunchKernelGGL(k_axpy4<double>, grid, dim3(bs), 0, st, n, (double*)out,
+                       (const double*)a0, c0, (const double*)a1, c1, (const double*)a2, c2,
+                       (const double*)a3, c3);
+}
+
+void launch_to_float(long long n, float* out, const double* in, hipStream_t st) {
+  const int bs = 256;
+  hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0,
+                     st, n, out, in);
+}
+
+void launch_to_double(long long n, double* out, const float* in, hipStream_t st) {
+  const int bs = 256;
+  hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0,
+                     st, n, out, in);
 }
 
 }  // namespace sac

@@ -18,8 +18,9 @@ LIB_PATH = os.path.join(HERE, "libsac_amd.so")
 
 SLOT_W, SLOT_S1, SLOT_S2, SLOT_ACC = 0, 1, 2, 3
 K_FINAL, K_AXIS, K_GHOST, K_CMAX, K_COPY = 0, 1, 2, 3, 4
+K_AXIS1, K_AXIS2, K_AXIS3 = 5, 6, 7
 ADV = {"onestep": 1, "fourstep": 4}
-PREC = {"fp64": 0}
+PREC = {"fp64": 0, "mixed": 1}
 
 EXPORTS = [
     "sac_dev_init", "sac_dev_free", "sac_dev_upload", "sac_dev_download",

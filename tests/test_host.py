@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from sac_amd import problems, vacio
-from tests.helpers import oracle_run, per_var_errors
+from tests.helpers import make_oracle, oracle_run, per_var_errors
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "host", "_build", "sac_vac")
@@ -93,7 +93,11 @@ def test_host_refuses_unbuilt_methods(exe, tmp_path, edit, msg):
 
 
 @pytest.mark.gpu
-def test_host_briowu_ascii_in_binary_out(exe, tmp_path, oracle_mod):
+def test_host_briowu_ascii_in_binary_out(exe, tmp_path):
+    """The program's run equals the library's own main loop bit for bit
+    (same calls, same dt), and the snapshots and log land where and when
+    the save settings ask."""
+    from sac_amd import device
     prob = problems.brio_wu(n=200)
     steps = 10
     par = _setup(tmp_path, prob, fmt="ascii", out_fmt="binary", itmax=steps, ditsave=(1, 5))
@@ -101,15 +105,32 @@ def test_host_briowu_ascii_in_binary_out(exe, tmp_path, oracle_mod):
     assert r.returncode == 0, r.stdout + r.stderr
     snaps = vacio.read_binary(tmp_path / "out.binary")
     assert [s.it for s in snaps] == [0, 5, 10]
-    ref, rdts, _ = oracle_run(prob, oracle_mod, steps)
+    w, dts = device.run(prob, steps=steps)
+    np.testing.assert_array_equal(snaps[-1].w, w[(Ellipsis,) + prob.mesh_slices()])
+    assert snaps[-1].t == sum(dts)
+    hdr, cols, rows = vacio.read_log(tmp_path / "run.log")
+    assert rows.shape[0] == steps + 1 and rows[-1, 0] == steps
+    np.testing.assert_allclose(rows[1:, 2], dts, rtol=1e-5)  # log precision: 6 digits
+
+
+@pytest.mark.gpu
+def test_host_briowu_fixed_dt_matches_oracle(exe, tmp_path, oracle_mod):
+    """With a fixed dt (dtpar) the run is compared with the oracle itself."""
+    prob = problems.brio_wu(n=200)
+    steps, dt = 20, 2.0e-4
+    par = _setup(tmp_path, prob, fmt="binary", out_fmt="binary", itmax=steps, dtpar=dt)
+    r = _run(exe, tmp_path, par)
+    assert r.returncode == 0, r.stdout + r.stderr
+    snaps = vacio.read_binary(tmp_path / "out.binary")
+    assert snaps[-1].it == steps
+    o = make_oracle(prob, oracle_mod)
+    for _ in range(steps):
+        o.advance(dt)
+    ref = o.get_w()
     got = prob.w.copy()
     got[(Ellipsis,) + prob.mesh_slices()] = snaps[-1].w
     errs = per_var_errors(prob, got, ref, 6)
     assert max(errs) <= 1e-12, errs
-    assert snaps[-1].t == pytest.approx(sum(rdts), rel=1e-12)
-    hdr, cols, rows = vacio.read_log(tmp_path / "run.log")
-    assert rows.shape[0] == steps + 1 and rows[-1, 0] == steps
-    assert rows[1, 2] == pytest.approx(rdts[0], rel=1e-5)  # log precision: 6 digits
 
 
 @pytest.mark.gpu
