@@ -153,6 +153,43 @@ int sac_dev_memory(sac_handle h, long long *bytes);
 /* Message of the last failure on this thread ("" if none). */
 const char *sac_dev_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * Ensembles of independent 1-D tubes (ndim=1, ndir=3, nw=13, 'cont'
+ * boundaries, 'fourstep').  No reference counterpart as one call: each tube
+ * is one serial run of the reference's main loop (src/vac.t:55-106) with its
+ * own dt (getdt_courant + getdt_visc), t and it, stopping at the first
+ * t >= tmax (or it >= itmax).  The ensemble advances on the device in
+ * hipGraph replays of several steps.  A tube stops at t >= tmax or it >= itmax
+ * (state 1), at dt < dtmin = 1e-99 (state 3, the reference's third finish
+ * condition, reached when a tube blows up), or when dt is not a number
+ * (state 2).
+ * ---------------------------------------------------------------------- */
+typedef void *sac_batch_handle;
+
+/* ncell[ntubes] mesh cells per tube (>= 6).  w: per tube the reference array
+ * w(ncell+4, nw) column-major, tubes one after another.  dx: per tube ncell+4
+ * cell widths after gridsetup1.  eqpar as for sac_dev_init (gamma at 1, eta
+ * at 2 must be 0, hyperdiffusion switch at nu_). */
+int sac_batch_init(int ntubes, const int *ncell, int nw, const double *w, const double *dx,
+                   const double *eqpar, int neqpar, int nu_, double courantpar, double tmax,
+                   int itmax, sac_batch_handle *out);
+int sac_batch_free(sac_batch_handle h);
+/* Autotuner: times trial_steps steps for every launch shape (cells per
+ * thread ck in {1,2,4} x threads per block tb in {64,128,256}) from the
+ * current state, restores the state, keeps the fastest shape. */
+int sac_batch_tune(sac_batch_handle h, int trial_steps, int *ck, int *tb, double *ms_per_step);
+/* Advance every tube to its end, replaying a graph of steps_per_graph steps
+ * at most max_steps steps in total; returns the steps issued. */
+int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,
+                  long long *steps_done);
+/* Per tube: t, it and state (0 running, 1 finished, 2 failed, 3 dt < dtmin). */
+int sac_batch_status(sac_batch_handle h, double *t, int *it, int *state);
+/* The ensemble state in the layout of sac_batch_init's w. */
+int sac_batch_download(sac_batch_handle h, double *w);
+int sac_batch_memory(sac_batch_handle h, long long *bytes);
+const char *sac_batch_last_error(void);
+
+
 #ifdef __cplusplus
 }
 #endif

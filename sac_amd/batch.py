@@ -1,0 +1,127 @@
+"""ctypes binding of the tube-ensemble entry points of libsac_amd.so
+(sac_batch_*, include/sac_amd.h): many independent 1-D MHD tubes of uneven
+length advanced together on one GPU, each with its own dt, t and it.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import problems
+from .device import SacError, _dp, _ip, grid_dx, load_library
+
+EXPORTS = [
+    "sac_batch_init", "sac_batch_free", "sac_batch_tune", "sac_batch_run", "sac_batch_status",
+    "sac_batch_download", "sac_batch_memory", "sac_batch_last_error",
+]
+
+STATE = {0: "running", 1: "finished", 2: "failed", 3: "dt below dtmin"}
+
+_declared = False
+
+
+def _lib():
+    global _declared
+    L = load_library()
+    if not _declared:
+        I, D, V = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        sig = {
+            "sac_batch_init": (I, [I, ip, I, dp, dp, dp, I, I, D, D, I, ctypes.POINTER(V)]),
+            "sac_batch_free": (I, [V]),
+            "sac_batch_tune": (I, [V, I, ip, ip, dp]),
+            "sac_batch_run": (I, [V, I, ctypes.c_longlong, llp]),
+            "sac_batch_status": (I, [V, dp, ip, ip]),
+            "sac_batch_download": (I, [V, dp]),
+            "sac_batch_memory": (I, [V, llp]),
+            "sac_batch_last_error": (ctypes.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _declared = True
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise SacError(f"libsac_amd batch error {rc}: {_lib().sac_batch_last_error().decode()}")
+
+
+def riemann_ensemble(count=4096, nmin=200, nmax=2000, seed=4096, nu=1.0):
+    """The seeded tube ensemble as Problems (gamma 5/3, cont boundaries)."""
+    return [problems.riemann_tube(n, l, r, gamma=5.0 / 3.0, nu=nu)
+            for n, l, r in problems.tube_ensemble(count, nmin, nmax, seed)]
+
+
+class Ensemble:
+    """Independent 1-D tubes (ndim=1, ndir=3) on one device."""
+
+    def __init__(self, tubes, courantpar=0.3, tmax=0.1, itmax=1000000):
+        L = _lib()
+        self.tubes = tubes
+        self.ncell = np.ascontiguousarray([t.nx[0] for t in tubes], dtype=np.int32)
+        nw = tubes[0].nw
+        for t in tubes:
+            if t.ndim != 1 or t.ndir != 3:
+                raise SacError("tube ensembles are 1-D with ndir=3")
+        # per tube w(G, nw) column-major == prob.w[v, 0, 0, i] in C order
+        w = np.concatenate([np.ascontiguousarray(t.w.reshape(nw, -1)).ravel() for t in tubes])
+        dx = np.concatenate([grid_dx(t) for t in tubes])
+        eq = np.ascontiguousarray(tubes[0].eqpar, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(L.sac_batch_init(len(tubes), _ip(self.ncell), nw, _dp(w), _dp(dx), _dp(eq),
+                                len(eq), problems.NU, courantpar, tmax, itmax, ctypes.byref(h)))
+        self._h = h
+        self.nw = nw
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().sac_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, trial_steps=2):
+        ck, tb, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(_lib().sac_batch_tune(self._h, trial_steps, ctypes.byref(ck), ctypes.byref(tb),
+                                     ctypes.byref(ms)))
+        return ck.value, tb.value, ms.value
+
+    def run(self, steps_per_graph=32, max_steps=10 ** 7):
+        n = ctypes.c_longlong()
+        _check(_lib().sac_batch_run(self._h, steps_per_graph, max_steps, ctypes.byref(n)))
+        return n.value
+
+    def status(self):
+        k = len(self.tubes)
+        t = np.zeros(k)
+        it = np.zeros(k, dtype=np.int32)
+        st = np.zeros(k, dtype=np.int32)
+        _check(_lib().sac_batch_status(self._h, _dp(t), _ip(it), _ip(st)))
+        return t, it, st
+
+    def download(self):
+        """Final state of every tube, shaped like its Problem's w."""
+        total = int(sum((n + 4) * self.nw for n in self.ncell))
+        w = np.zeros(total)
+        _check(_lib().sac_batch_download(self._h, _dp(w)))
+        out, off = [], 0
+        for t in self.tubes:
+            sz = t.w.size
+            out.append(w[off:off + sz].reshape(t.w.shape))
+            off += sz
+        return out
+
+    def memory(self):
+        b = ctypes.c_longlong()
+        _check(_lib().sac_batch_memory(self._h, ctypes.byref(b)))
+        return b.value

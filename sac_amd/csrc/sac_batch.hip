@@ -1,0 +1,682 @@
+// sac_batch.hip -- ensembles of independent 1-D MHD tubes of uneven length
+// advanced together on one GPU.
+//
+// Every tube is its own serial run of the reference (src/vac.t:55-106): its
+// own dt from getdt_courant and getdt_special (lagged maxviscoef, forced
+// dtdiffpar 0.25), its own t and it, 'cont' boundaries, classical RK4
+// ('fourstep').  The tubes share the kernels: one launch per kernel covers
+// every tube through a tile table (tube, first mesh cell), tiles never
+// straddle two tubes, so the per-tube reductions are block reductions plus
+// one atomic per block.  The time loop runs on the device: a small kernel
+// computes each tube's next dt and finish flag, the RK stages use it, and a
+// second small kernel advances t and it.  One step is a fixed sequence of
+// launches, captured once in a hipGraph holding several steps and replayed
+// until every tube has reached tmax; finished tubes skip their tiles.
+//
+// The line work and the combining update are the grid path's own device
+// functions (axis_group, final_cell) applied to a per-tube 1-D grid
+// descriptor, so a tube advanced here follows the same arithmetic as the
+// same tube run alone through sac_dev_*.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sac_amd.h"
+#include "sac_device.h"
+
+namespace sac {
+namespace batch {
+
+constexpr int NDIR = 3;
+constexpr int NE = 2 + 2 * NDIR;  // evolving fields
+constexpr int NBG = 2 + NDIR;     // background fields
+constexpr int kSlots = 4;         // W, S1, S2, ACC
+constexpr int kRed = 2;           // per tube and register: max cmax, max cmax/dx
+
+// FINISHED: t >= tmax or it >= itmax; STOPPED: dt < dtmin (the reference's
+// third finish condition, src/vac.t:591, hit when a tube's state blows up
+// and cmax or maxviscoef become infinite); FAILED: dt not a number
+enum TubeState : int { RUNNING = 0, FINISHED = 1, FAILED = 2, STOPPED = 3 };
+
+struct Tile {
+  int tube;  // tube index
+  int c0;    // first mesh cell (grid index) of the tile
+};
+
+struct Args {
+  int ntubes;
+  long long fs;              // field stride (elements)
+  const long long* start;    // grid origin of each tube in the packed layout
+  const int* ncell;          // mesh cells per tube
+  const Tile* tiles;
+  int ntiles;
+  double* reg[kSlots];
+  const double* bg;
+  const double* dx;          // cell widths, packed like one field
+  double* R;
+  unsigned long long* red;   // [tube][slot][kRed]
+  unsigned long long* visc;  // [tube] maxviscoef since the last dt
+  double* t;
+  double* dt;
+  int* it;
+  int* state;
+  const double* dxmin;       // per tube, over its mesh
+  double gamma;
+  int visc_on;
+  double courantpar, tmax, dtmin;
+  int itmax;
+  unsigned int* running;     // count of tubes still running after a step
+};
+
+__device__ __forceinline__ GridDesc tube_grid(const Args& a, int tube) {
+  GridDesc g{};
+  g.ndim = 1;
+  g.ndir = NDIR;
+  const int n = a.ncell[tube];
+  g.G[0] = n + 4;
+  g.G[1] = g.G[2] = 1;
+  g.Mlo[0] = 2;
+  g.Mhi[0] = n + 1;
+  g.st[0] = 1;
+  g.st[1] = g.st[2] = g.G[0];
+  g.fs = a.fs;
+  return g;
+}
+
+__device__ __forceinline__ unsigned long long* red_of(const Args& a, int tube, int slot) {
+  return a.red + ((size_t)tube * kSlots + slot) * kRed;
+}
+
+// line work of one tile: each thread one group of CK cells
+template <int CK>
+__global__ void __launch_bounds__(256) kb_axis(Args a, int wct) {
+  const Tile tl = a.tiles[blockIdx.x];
+  if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
+  const GridDesc g = tube_grid(a, tl.tube);
+  const long long o = a.start[tl.tube];
+  AxisArgs x{};
+  x.g = g;
+  x.prec = PREC_FP64;
+  x.wct = a.reg[wct] + o;
+  x.bg = a.bg + o;
+  x.dx[0] = a.dx + o;
+  x.R = a.R + o;
+  x.T = nullptr;
+  x.gamma = a.gamma;
+  x.visc_on = a.visc_on;
+  x.ctot_src = red_of(a, tl.tube, wct);
+  x.red_visc = a.visc + tl.tube;
+  double numax = 0.0;
+  int c[3] = {tl.c0 + (int)threadIdx.x * CK, 0, 0};
+  if (c[0] <= g.Mhi[0]) axis_group<1, NDIR, 0, CK, double, double>(x, g, c, numax);
+  if (a.visc_on) {
+    double r[1] = {numax};
+    block_max_atomic<1>(r, x.red_visc);
+  }
+}
+
+// combining update of one tile: out = base + dt/da * R, acc_out = acc_in + dt/db * R
+template <int CK>
+__global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int out, double da,
+                                                int acc_in, int acc_out, double db) {
+  const Tile tl = a.tiles[blockIdx.x];
+  if (a.state[tl.tube] != RUNNING) return;
+  const GridDesc g = tube_grid(a, tl.tube);
+  const long long o = a.start[tl.tube];
+  const double dt = a.dt[tl.tube];
+  FinalArgs f{};
+  f.g = g;
+  f.prec = PREC_FP64;
+  f.wct = a.reg[wct] + o;
+  f.bg = a.bg + o;
+  f.dx[0] = a.dx + o;
+  f.R = a.R + o;
+  f.T = nullptr;
+  f.base = a.reg[base] + o;
+  f.out = a.reg[out] + o;
+  f.acc_in = acc_in >= 0 ? a.reg[acc_in] + o : nullptr;
+  f.acc_out = acc_out >= 0 ? a.reg[acc_out] + o : nullptr;
+  f.qa = dt / da;
+  f.qb = acc_out >= 0 ? dt / db : 0.0;
+  f.gamma = a.gamma;
+  f.visc_on = a.visc_on;
+  double redv[2] = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    int c[3] = {tl.c0 + (int)threadIdx.x * CK + j, 0, 0};
+    if (c[0] <= g.Mhi[0]) {
+      double r[2] = {0.0, 0.0};
+      final_cell<1, NDIR, double, double>(f, g, c, r);
+      redv[0] = fmax(redv[0], r[0]);
+      redv[1] = fmax(redv[1], r[1]);
+    }
+  }
+  block_max_atomic<2>(redv, red_of(a, tl.tube, out));
+}
+
+// cmax reductions of register `slot` of every running tube
+template <int CK>
+__global__ void __launch_bounds__(256) kb_cmax(Args a, int slot) {
+  const Tile tl = a.tiles[blockIdx.x];
+  if (a.state[tl.tube] != RUNNING) return;
+  const GridDesc g = tube_grid(a, tl.tube);
+  const long long o = a.start[tl.tube];
+  double redv[2] = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    const int c0 = tl.c0 + (int)threadIdx.x * CK + j;
+    if (c0 <= g.Mhi[0]) {
+      Pt<NDIR, double> p;
+      load_pt<NDIR, double, double>(a.reg[slot] + o, a.bg + o, a.fs, c0, p);
+      double cm[1];
+      cmax_all<1, NDIR>(p, a.gamma, cm);
+      redv[0] = fmax(redv[0], cm[0]);
+      redv[1] = fmax(redv[1], cm[0] / a.dx[o + c0]);
+    }
+  }
+  block_max_atomic<2>(redv, red_of(a, tl.tube, slot));
+}
+
+// 'cont' ghost cells of nfield packed fields f: 2 per side, copies of the
+// edge cell (getboundary, src/vacgrid.t:366-370)
+__global__ void kb_ghost(Args a, double* f, int nfield) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int tube = (int)(i >> 2), k = (int)(i & 3);
+  if (tube >= a.ntubes || a.state[tube] != RUNNING) return;
+  const int n = a.ncell[tube];
+  const long long o = a.start[tube];
+  const int cg = (k < 2) ? k : n + k;  // ghost cells 0,1 and n+2,n+3
+  const int ce = (k < 2) ? 2 : n + 1;  // edge mesh cell
+  for (int v = 0; v < nfield; v++) f[v * a.fs + o + cg] = f[v * a.fs + o + ce];
+}
+
+// zero the reductions of register `slot` of every tube
+__global__ void kb_zero_red(Args a, int slot) {
+  const int tube = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tube >= a.ntubes) return;
+  unsigned long long* r = red_of(a, tube, slot);
+#pragma unroll
+  for (int k = 0; k < kRed; k++) r[k] = 0ull;
+}
+
+__device__ __forceinline__ double red_val(const unsigned long long* p) {
+  return __longlong_as_double((long long)*p);
+}
+
+// dt and finish check of every running tube (src/vac.t:63-81):
+// getdt_courant (src/vac.t:641-699), getdt_visc (src/vacusr.viscosity.t:509-541,
+// dtdiffpar 0.25, maxviscoef reset), then timetofinish (src/vac.t:585-605).
+// A tube whose dt is not a positive finite number is stopped as FAILED.
+__global__ void kb_dt(Args a) {
+  const int tube = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tube >= a.ntubes || a.state[tube] != RUNNING) return;
+  const double cm = red_val(red_of(a, tube, SAC_SLOT_W) + 1);
+  double dt = 1.0e99;
+  if (cm > 1.0e-99) dt = fmin(dt, a.courantpar / cm);
+  if (a.visc_on) {
+    const double mv = red_val(a.visc + tube);
+    const double dxm = a.dxmin[tube];
+    dt = fmin(dt, 0.25 / (mv / (dxm * dxm)));
+    a.visc[tube] = 0ull;
+  }
+  if (a.it[tube] >= a.itmax || a.t[tube] >= a.tmax) {
+    a.state[tube] = FINISHED;
+  } else if (dt < a.dtmin) {
+    a.state[tube] = STOPPED;
+  } else if (!(dt > 0.0) || !isfinite(dt)) {
+    a.state[tube] = FAILED;
+  } else {
+    a.dt[tube] = dt;
+  }
+}
+
+// t += dt, it += 1 after the step; counts the tubes still running
+__global__ void kb_tick(Args a) {
+  const int tube = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tube >= a.ntubes || a.state[tube] != RUNNING) return;
+  a.t[tube] += a.dt[tube];
+  a.it[tube] += 1;
+  atomicAdd(a.running, 1u);
+}
+
+__global__ void kb_zero_u32(unsigned int* p) { *p = 0u; }
+
+}  // namespace batch
+}  // namespace sac
+
+namespace {
+
+using namespace sac::batch;
+
+thread_local std::string g_berr;
+
+int bfail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_berr = buf;
+  return code;
+}
+
+#define BCHK(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return bfail(SAC_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// launch shapes the autotuner chooses from: cells per thread (CK) and
+// threads per block (TB); a tile is TB*CK consecutive cells of one tube
+struct Shape {
+  int ck, tb;
+};
+const Shape kShapes[] = {{1, 64}, {1, 128}, {1, 256}, {2, 64}, {2, 128},
+                         {2, 256}, {4, 64}, {4, 128}, {4, 256}};
+
+struct Batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int ntubes = 0;
+  std::vector<int> ncell;
+  std::vector<long long> start;  // packed grid origins
+  long long fs = 0;
+  Args a{};
+  // device buffers
+  long long* d_start = nullptr;
+  int* d_ncell = nullptr;
+  Tile* d_tiles = nullptr;
+  double* d_reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+  double* d_bg = nullptr;
+  double* d_dx = nullptr;
+  double* d_R = nullptr;
+  double* d_save = nullptr;  // state kept aside while the autotuner runs
+  unsigned long long* d_red = nullptr;
+  unsigned long long* d_visc = nullptr;
+  double* d_t = nullptr;
+  double* d_dt = nullptr;
+  int* d_it = nullptr;
+  int* d_state = nullptr;
+  double* d_dxmin = nullptr;
+  unsigned int* d_running = nullptr;
+  Shape shape{2, 128};
+  int ntiles = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  long long bytes = 0;
+  long long steps_run = 0;
+};
+
+int balloc(Batch& b, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return bfail(SAC_E_NOMEM, "hipMalloc of %zu bytes: %s", bytes,
+                                    hipGetErrorString(e));
+  b.bytes += (long long)bytes;
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) return bfail(SAC_E_HIP, "hipMemset: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int set_shape(Batch& b, Shape s) {
+  std::vector<Tile> tiles;
+  const int span = s.ck * s.tb;
+  for (int i = 0; i < b.ntubes; i++)
+    for (int c = 0; c < b.ncell[i]; c += span) tiles.push_back({i, 2 + c});
+  if (b.d_tiles) hipFree(b.d_tiles);
+  b.d_tiles = nullptr;
+  BCHK(hipMalloc((void**)&b.d_tiles, sizeof(Tile) * tiles.size()));
+  BCHK(hipMemcpy(b.d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+  b.ntiles = (int)tiles.size();
+  b.shape = s;
+  b.a.tiles = b.d_tiles;
+  b.a.ntiles = b.ntiles;
+  if (b.graph) {
+    hipGraphExecDestroy(b.graph);
+    b.graph = nullptr;
+  }
+  return 0;
+}
+
+template <int CK>
+void enqueue_step_ck(Batch& b) {
+  const Args& a = b.a;
+  hipStream_t st = b.stream;
+  const dim3 gt(b.ntiles), bt(b.shape.tb);
+  const dim3 gn((b.ntubes + 255) / 256), bn(256);
+  const dim3 gg((unsigned)((4LL * b.ntubes + 255) / 256));
+  const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, A = SAC_SLOT_ACC;
+  hipLaunchKernelGGL(kb_dt, gn, bn, 0, st, a);
+  // (wct, base, out, da, acc_in, acc_out, db) of the four RK4 stages, as in
+  // the grid path: S1=W+dt/2 k1, A=W+dt/6 k1; S2=W+dt/2 k2, A+=dt/3 k2;
+  // S1=W+dt k3, A+=dt/3 k3; W=A+dt/6 k4
+  struct St {
+    int wct, base, out;
+    double da;
+    int ai, ao;
+    double db;
+  };
+  const St stages[4] = {{W, W, S1, 2.0, W, A, 6.0},
+                        {S1, W, S2, 2.0, A, A, 3.0},
+                        {S2, W, S1, 1.0, A, A, 3.0},
+                        {S1, A, W, 6.0, -1, -1, 0.0}};
+  for (const St& s : stages) {
+    hipLaunchKernelGGL(kb_axis<CK>, gt, bt, 0, st, a, s.wct);
+    hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, st, a, s.out);
+    hipLaunchKernelGGL(kb_final<CK>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai, s.ao,
+                       s.db);
+    hipLaunchKernelGGL(kb_ghost, gg, dim3(256), 0, st, a, a.reg[s.out], NE);
+  }
+  hipLaunchKernelGGL(kb_tick, gn, bn, 0, st, a);
+}
+
+void enqueue_step(Batch& b) {
+  switch (b.shape.ck) {
+    case 1: enqueue_step_ck<1>(b); break;
+    case 2: enqueue_step_ck<2>(b); break;
+    default: enqueue_step_ck<4>(b); break;
+  }
+}
+
+void enqueue_cmax(Batch& b, int slot) {
+  const dim3 gn((b.ntubes + 255) / 256), bn(256);
+  hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, b.stream, b.a, slot);
+  const dim3 gt(b.ntiles), bt(b.shape.tb);
+  switch (b.shape.ck) {
+    case 1: hipLaunchKernelGGL(kb_cmax<1>, gt, bt, 0, b.stream, b.a, slot); break;
+    case 2: hipLaunchKernelGGL(kb_cmax<2>, gt, bt, 0, b.stream, b.a, slot); break;
+    default: hipLaunchKernelGGL(kb_cmax<4>, gt, bt, 0, b.stream, b.a, slot); break;
+  }
+}
+
+// capture `steps` time steps (plus the running-tube counter reset) in one graph
+int build_graph(Batch& b, int steps) {
+  if (b.graph) {
+    hipGraphExecDestroy(b.graph);
+    b.graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  BCHK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(kb_zero_u32, dim3(1), dim3(1), 0, b.stream, b.a.running);
+  for (int s = 0; s < steps; s++) enqueue_step(b);
+  BCHK(hipStreamEndCapture(b.stream, &g));
+  hipError_t e = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return bfail(SAC_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  b.graph_steps = steps;
+  return 0;
+}
+
+// reset the per-tube run state (t, it, state, maxviscoef) from host values
+int reset_state(Batch& b, const std::vector<double>& t0) {
+  std::vector<int> zi(b.ntubes, 0);
+  BCHK(hipMemcpy(b.d_t, t0.data(), sizeof(double) * b.ntubes, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b.d_it, zi.data(), sizeof(int) * b.ntubes, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b.d_state, zi.data(), sizeof(int) * b.ntubes, hipMemcpyHostToDevice));
+  BCHK(hipMemset(b.d_visc, 0, sizeof(unsigned long long) * b.ntubes));
+  BCHK(hipMemset(b.d_dt, 0, sizeof(double) * b.ntubes));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sac_batch_last_error(void) { return g_berr.c_str(); }
+
+int sac_batch_init(int ntubes, const int* ncell, int nw, const double* w, const double* dx,
+                   const double* eqpar, int neqpar, int nu_, double courantpar, double tmax,
+                   int itmax, sac_batch_handle* out) {
+  g_berr.clear();
+  if (!out) return bfail(SAC_E_ARG, "out is null");
+  *out = nullptr;
+  if (ntubes < 1 || !ncell || !w || !dx || !eqpar) return bfail(SAC_E_ARG, "bad arguments");
+  if (nw != NE + NBG) return bfail(SAC_E_UNSUPPORTED, "tubes carry nw=%d variables (ndir=3)", NE + NBG);
+  if (neqpar < 2 || eqpar[0] <= 0.0) return bfail(SAC_E_ARG, "gamma must be > 0");
+  if (eqpar[1] != 0.0) return bfail(SAC_E_UNSUPPORTED, "resistive MHD (eta /= 0) is not built");
+  if (!(courantpar > 0.0) || !(tmax > 0.0) || itmax < 1)
+    return bfail(SAC_E_ARG, "courantpar, tmax and itmax must be positive");
+  for (int i = 0; i < ntubes; i++)
+    if (ncell[i] < 6) return bfail(SAC_E_ARG, "tube %d has %d < 6 cells", i, ncell[i]);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return bfail(SAC_E_HIP, "no HIP device available");
+  Batch* b = new Batch();
+  auto bail = [&](int code) {
+    sac_batch_free(b);
+    return code;
+  };
+  hipGetDevice(&b->device);
+  b->ntubes = ntubes;
+  b->ncell.assign(ncell, ncell + ntubes);
+  b->start.resize(ntubes);
+  long long off = 0;
+  for (int i = 0; i < ntubes; i++) {
+    b->start[i] = off;
+    off += (ncell[i] + 4 + 15) / 16 * 16;  // each tube starts on a 128-byte line
+  }
+  b->fs = (off + 63) / 64 * 64;
+  if (hipStreamCreate(&b->stream) != hipSuccess) return bail(bfail(SAC_E_HIP, "hipStreamCreate"));
+  int rc = 0;
+  const size_t fb = sizeof(double) * (size_t)b->fs;
+  for (int s = 0; s < kSlots; s++)
+    if ((rc = balloc(*b, (void**)&b->d_reg[s], fb * NE))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_bg, fb * NBG))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_R, fb * NE))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_dx, fb))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_start, sizeof(long long) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_ncell, sizeof(int) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_red, sizeof(unsigned long long) * ntubes * kSlots * kRed)))
+    return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_visc, sizeof(unsigned long long) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_t, sizeof(double) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_dt, sizeof(double) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_it, sizeof(int) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_state, sizeof(int) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_dxmin, sizeof(double) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_running, sizeof(unsigned int)))) return bail(rc);
+  // host layout per tube: w(G_i, nw) column-major, tubes one after another;
+  // dx: G_i values per tube
+  std::vector<double> pk((size_t)b->fs * (NE + NBG), 0.0), pdx((size_t)b->fs, 1.0);
+  std::vector<double> dxmin(ntubes);
+  long long hoff = 0, hdx = 0;
+  for (int i = 0; i < ntubes; i++) {
+    const int G = ncell[i] + 4;
+    for (int v = 0; v < NE + NBG; v++)
+      memcpy(&pk[(size_t)v * b->fs + b->start[i]], w + hoff + (size_t)v * G, sizeof(double) * G);
+    memcpy(&pdx[b->start[i]], dx + hdx, sizeof(double) * G);
+    double mn = 1.0e99;
+    for (int c = 2; c <= ncell[i] + 1; c++) {
+      if (!(dx[hdx + c] > 0.0)) return bail(bfail(SAC_E_ARG, "dx must be > 0 (tube %d)", i));
+      mn = std::min(mn, dx[hdx + c]);
+    }
+    dxmin[i] = mn;
+    hoff += (long long)G * (NE + NBG);
+    hdx += G;
+  }
+  if (hipMemcpy(b->d_reg[SAC_SLOT_W], pk.data(), fb * NE, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_bg, pk.data() + (size_t)b->fs * NE, fb * NBG, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_dx, pdx.data(), fb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_start, b->start.data(), sizeof(long long) * ntubes, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_ncell, ncell, sizeof(int) * ntubes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_dxmin, dxmin.data(), sizeof(double) * ntubes, hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return bail(bfail(SAC_E_HIP, "upload failed"));
+  Args& a = b->a;
+  a.ntubes = ntubes;
+  a.fs = b->fs;
+  a.start = b->d_start;
+  a.ncell = b->d_ncell;
+  for (int s = 0; s < kSlots; s++) a.reg[s] = b->d_reg[s];
+  a.bg = b->d_bg;
+  a.dx = b->d_dx;
+  a.R = b->d_R;
+  a.red = b->d_red;
+  a.visc = b->d_visc;
+  a.t = b->d_t;
+  a.dt = b->d_dt;
+  a.it = b->d_it;
+  a.state = b->d_state;
+  a.dxmin = b->d_dxmin;
+  a.gamma = eqpar[0];
+  const double nu = (nu_ >= 1 && nu_ <= neqpar) ? eqpar[nu_ - 1] : 0.0;
+  a.visc_on = std::fabs(nu) > 1.0e-99;
+  a.courantpar = courantpar;
+  a.tmax = tmax;
+  a.dtmin = 1.0e-99;
+  a.itmax = itmax;
+  a.running = b->d_running;
+  if ((rc = set_shape(*b, Shape{2, 128}))) return bail(rc);
+  // initial getboundary (background and state) and cmax of W
+  const dim3 gg((unsigned)((4LL * ntubes + 255) / 256));
+  hipLaunchKernelGGL(kb_ghost, gg, dim3(256), 0, b->stream, a, b->d_bg, NBG);
+  hipLaunchKernelGGL(kb_ghost, gg, dim3(256), 0, b->stream, a, b->d_reg[SAC_SLOT_W], NE);
+  enqueue_cmax(*b, SAC_SLOT_W);
+  if (hipStreamSynchronize(b->stream) != hipSuccess)
+    return bail(bfail(SAC_E_HIP, "initial fill: %s", hipGetErrorString(hipGetLastError())));
+  *out = b;
+  return 0;
+}
+
+int sac_batch_free(sac_batch_handle h) {
+  if (!h) return 0;
+  Batch* b = (Batch*)h;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  for (int s = 0; s < kSlots; s++) hipFree(b->d_reg[s]);
+  hipFree(b->d_bg);
+  hipFree(b->d_dx);
+  hipFree(b->d_R);
+  hipFree(b->d_save);
+  hipFree(b->d_start);
+  hipFree(b->d_ncell);
+  hipFree(b->d_tiles);
+  hipFree(b->d_red);
+  hipFree(b->d_visc);
+  hipFree(b->d_t);
+  hipFree(b->d_dt);
+  hipFree(b->d_it);
+  hipFree(b->d_state);
+  hipFree(b->d_dxmin);
+  hipFree(b->d_running);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int sac_batch_tune(sac_batch_handle h, int trial_steps, int* ck, int* tb, double* ms_per_step) {
+  if (!h) return bfail(SAC_E_ARG, "null handle");
+  Batch& b = *(Batch*)h;
+  if (trial_steps < 1) trial_steps = 2;
+  // keep the state, the reductions and the run counters aside
+  const size_t fb = sizeof(double) * (size_t)b.fs * NE;
+  if (!b.d_save) {
+    int rc = balloc(b, (void**)&b.d_save, fb);
+    if (rc) return rc;
+  }
+  std::vector<unsigned long long> red0((size_t)b.ntubes * kSlots * kRed);
+  std::vector<double> t0(b.ntubes);
+  BCHK(hipMemcpy(b.d_save, b.d_reg[SAC_SLOT_W], fb, hipMemcpyDeviceToDevice));
+  BCHK(hipMemcpy(red0.data(), b.d_red, sizeof(unsigned long long) * red0.size(),
+                 hipMemcpyDeviceToHost));
+  BCHK(hipMemcpy(t0.data(), b.d_t, sizeof(double) * b.ntubes, hipMemcpyDeviceToHost));
+  double best = 1e300;
+  Shape bs = b.shape;
+  for (const Shape& s : kShapes) {
+    if (int rc = set_shape(b, s)) return rc;
+    if (int rc = build_graph(b, trial_steps)) return rc;
+    BCHK(hipGraphLaunch(b.graph, b.stream));  // warm-up replay
+    BCHK(hipStreamSynchronize(b.stream));
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, b.stream);
+    BCHK(hipGraphLaunch(b.graph, b.stream));
+    hipEventRecord(e1, b.stream);
+    BCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    const double per = ms / trial_steps;
+    if (per < best) {
+      best = per;
+      bs = s;
+    }
+    // restore the state for the next candidate
+    BCHK(hipMemcpy(b.d_reg[SAC_SLOT_W], b.d_save, fb, hipMemcpyDeviceToDevice));
+    BCHK(hipMemcpy(b.d_red, red0.data(), sizeof(unsigned long long) * red0.size(),
+                   hipMemcpyHostToDevice));
+    if (int rc = reset_state(b, t0)) return rc;
+  }
+  if (int rc = set_shape(b, bs)) return rc;
+  if (ck) *ck = bs.ck;
+  if (tb) *tb = bs.tb;
+  if (ms_per_step) *ms_per_step = best;
+  return 0;
+}
+
+int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,
+                  long long* steps_done) {
+  if (!h) return bfail(SAC_E_ARG, "null handle");
+  Batch& b = *(Batch*)h;
+  if (steps_per_graph < 1) steps_per_graph = 32;
+  if (!b.graph || b.graph_steps != steps_per_graph) {
+    if (int rc = build_graph(b, steps_per_graph)) return rc;
+  }
+  long long done = 0;
+  while (done < max_steps) {
+    BCHK(hipGraphLaunch(b.graph, b.stream));
+    unsigned int running = 0;
+    BCHK(hipMemcpyAsync(&running, b.d_running, sizeof(running), hipMemcpyDeviceToHost, b.stream));
+    BCHK(hipStreamSynchronize(b.stream));
+    done += steps_per_graph;
+    if (running == 0) break;
+  }
+  b.steps_run += done;
+  if (steps_done) *steps_done = done;
+  return 0;
+}
+
+int sac_batch_status(sac_batch_handle h, double* t, int* it, int* state) {
+  if (!h) return bfail(SAC_E_ARG, "null handle");
+  Batch& b = *(Batch*)h;
+  if (t) BCHK(hipMemcpy(t, b.d_t, sizeof(double) * b.ntubes, hipMemcpyDeviceToHost));
+  if (it) BCHK(hipMemcpy(it, b.d_it, sizeof(int) * b.ntubes, hipMemcpyDeviceToHost));
+  if (state) BCHK(hipMemcpy(state, b.d_state, sizeof(int) * b.ntubes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int sac_batch_download(sac_batch_handle h, double* w) {
+  if (!h || !w) return bfail(SAC_E_ARG, "null argument");
+  Batch& b = *(Batch*)h;
+  BCHK(hipStreamSynchronize(b.stream));
+  const size_t fb = sizeof(double) * (size_t)b.fs;
+  std::vector<double> pk((size_t)b.fs * (NE + NBG));
+  BCHK(hipMemcpy(pk.data(), b.d_reg[SAC_SLOT_W], fb * NE, hipMemcpyDeviceToHost));
+  BCHK(hipMemcpy(pk.data() + (size_t)b.fs * NE, b.d_bg, fb * NBG, hipMemcpyDeviceToHost));
+  long long hoff = 0;
+  for (int i = 0; i < b.ntubes; i++) {
+    const int G = b.ncell[i] + 4;
+    for (int v = 0; v < NE + NBG; v++)
+      memcpy(w + hoff + (size_t)v * G, &pk[(size_t)v * b.fs + b.start[i]], sizeof(double) * G);
+    hoff += (long long)G * (NE + NBG);
+  }
+  return 0;
+}
+
+int sac_batch_memory(sac_batch_handle h, long long* bytes) {
+  if (!h) return bfail(SAC_E_ARG, "null handle");
+  if (bytes) *bytes = ((Batch*)h)->bytes;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,674 @@
+// sac_device.h -- device-side building blocks of the SAC perturbation-MHD
+// stage shared by the grid kernels (sac_kernels.hip) and the batched tube
+// kernels (sac_batch.hip): point state, fluxes, cmax, setnu face ratios,
+// the per-axis line-group work, the per-cell combining update and the
+// ghost-value chain.
+//
+// Every kernel is a template over the storage type S of the evolving fields
+// (and of the rate and cross-term scratch) and the compute type C of the
+// flux and hyperdiffusion arithmetic:
+//   FP64:  S = C = double;
+//   mixed: S = C = float, with FP64 kept for the background fields, the
+//          pressure recovery from the split total energy (getptotal and the
+//          thermal energy of setnu), the register update, cmax and every
+//          reduction; the flux differences of gradient4 are compensated.
+//
+// Reference routines restated here:
+//   getflux            src/vacphys.mhd0.t:147-222
+//   getptotal(_bg)     src/vacphys0.t.mhd:186-224
+//   getcmax            src/vacphys0.t.mhd:77-112
+//   gradient4          src/vacgrid.t:665-712
+//   centdiff4          src/vaccd.t:7-78
+//   addsource_grav     src/vacusr.gravity.t:24-81
+//   addsource_visc     src/vacusr.viscosity.t:2-221
+//   setnu              src/vacusr.viscosity.t:224-455
+//   gradient1/1L/1R    src/vacusr.viscosity.t:547-675
+//   getboundary        src/vacgrid.t:195-512
+#pragma once
+#include "sac_kernels.h"
+
+// mesh cells per thread of the line kernels along x and along y/z
+#ifndef SAC_CK_X
+#define SAC_CK_X 2
+#endif
+#ifndef SAC_CK_YZ
+#define SAC_CK_YZ 2
+#endif
+
+// keeps the machine scheduler from hoisting the next phase's loads above
+// the current phase's arithmetic, which bounds the number of live values
+#define SCHED_FENCE()                  \
+  do {                                 \
+    asm volatile("" ::: "memory");     \
+    __builtin_amdgcn_sched_barrier(0); \
+  } while (0)
+
+namespace sac {
+
+// ---------------------------------------------------------------------
+// precision helpers
+
+// a/b for 0 <= a <= 6b, b > 0 (the setnu ratio md3/md1): IEEE division in
+// FP64; in FP32 a reciprocal-multiply, with both operands scaled by 2^100
+// (exact) when b is below 2^-100 so that the reciprocal of a denormal never
+// overflows
+__device__ __forceinline__ double cdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float cdiv(float a, float b) {
+  const float s = (b < 0x1p-100f) ? 0x1p100f : 1.0f;
+  return (a * s) * __builtin_amdgcn_rcpf(b * s);
+}
+
+// 1/x in FP64 for the pressure recovery: IEEE division in FP64 mode; in
+// mixed mode the FP32 reciprocal refined by one Newton step in FP64
+// (relative error ~1e-14)
+template <typename C>
+__device__ __forceinline__ double recip(double x);
+template <>
+__device__ __forceinline__ double recip<double>(double x) {
+  return 1.0 / x;
+}
+template <>
+__device__ __forceinline__ double recip<float>(double x) {
+  const double r = (double)__builtin_amdgcn_rcpf((float)x);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+
+// a-b with its rounding error (FP64 mode: error 0, folded away)
+__device__ __forceinline__ double two_diff(double a, double b, double& err) {
+  err = 0.0;
+  return a - b;
+}
+__device__ __forceinline__ float two_diff(float a, float b, float& err) {
+  const float s = a - b;
+  const float bb = s - a;
+  err = (a - (s - bb)) - (b + bb);
+  return s;
+}
+
+// ---------------------------------------------------------------------
+// point state: perturbation (evolving) in C, static background in FP64
+
+template <int NDIR, typename C>
+struct Pt {
+  C rho, m[NDIR], e, b[NDIR];
+  double eb, rhob, bg[NDIR];
+};
+
+template <int NDIR, typename S, typename C>
+__device__ __forceinline__ void load_pt(const S* __restrict__ W, const double* __restrict__ B,
+                                        long long fs, long long i, Pt<NDIR, C>& p) {
+  p.rho = (C)W[i];
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) p.m[c] = (C)W[(1 + c) * fs + i];
+  p.e = (C)W[(1 + NDIR) * fs + i];
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) p.b[c] = (C)W[(2 + NDIR + c) * fs + i];
+  p.eb = B[i];
+  p.rhob = B[fs + i];
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) p.bg[c] = B[(2 + c) * fs + i];
+}
+
+// perturbation total pressure p_T' in FP64 (getptotal, src/vacphys0.t.mhd:186-205);
+// inv = 1/rho_t
+template <int NDIR, typename C>
+__device__ __forceinline__ double ptotal(const Pt<NDIR, C>& p, double gamma, double inv) {
+  double sbbg = 0.0, sb2 = 0.0, sm2 = 0.0;
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) {
+    const double b = (double)p.b[c], m = (double)p.m[c];
+    sbbg += b * p.bg[c];
+    sb2 += b * b;
+    sm2 += m * m;
+  }
+  return (gamma - 1.0) * ((double)p.e - 0.5 * sm2 * inv) - (gamma - 2.0) * (sbbg + 0.5 * sb2);
+}
+
+// background total pressure (getptotal_bg, src/vacphys0.t.mhd:208-224)
+template <int NDIR, typename C>
+__device__ __forceinline__ double ptotal_bg(const Pt<NDIR, C>& p, double gamma) {
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) s += p.bg[c] * p.bg[c];
+  return (gamma - 1.0) * p.eb - 0.5 * (gamma - 2.0) * s;
+}
+
+// full flux (non-transport part of getflux + transport v_d*w) along axis d;
+// also returns v_l = m_l/rho_t for l < NV
+template <int NDIR, int NV, typename C>
+__device__ __forceinline__ void flux_pt(const Pt<NDIR, C>& p, int d, double gamma,
+                                        C (&F)[2 + 2 * NDIR], C (&v)[NV]) {
+  const double rt = (double)p.rho + p.rhob;
+  const double invd = recip<C>(rt);
+  const C inv = (C)invd;
+  const C vd = p.m[d] * inv;
+  F[0] = (C)p.rhob * p.m[d] * inv + vd * p.rho;
+  const C pT = (C)ptotal<NDIR, C>(p, gamma, invd);
+  C sbgm = 0, sbm = 0;
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) {
+    sbgm += (C)p.bg[c] * p.m[c];
+    sbm += p.b[c] * p.m[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) {
+    C f = -(p.b[c] * (C)p.bg[d] + p.b[d] * (C)p.bg[c]) - p.b[c] * p.b[d];
+    if (c == d) f = pT + f;
+    F[1 + c] = f + vd * p.m[c];
+  }
+  F[1 + NDIR] = (p.m[d] * pT - p.b[d] * sbgm - (C)p.bg[d] * sbm) * inv +
+                (C)p.eb * p.m[d] * inv - p.b[d] * sbm * inv + vd * p.e;
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) {
+    F[2 + NDIR + c] = (c == d) ? (C)0
+                               : (-p.m[c] * inv * (p.b[d] + (C)p.bg[d]) +
+                                  p.m[d] * inv * (C)p.bg[c] + vd * p.b[c]);
+  }
+#pragma unroll
+  for (int l = 0; l < NV; l++) v[l] = p.m[l] * inv;
+}
+
+// fast-mode speed + |v_d| (getcmax, src/vacphys0.t.mhd:77-112) in FP64 for
+// every direction d < NDIM; the direction-independent part (c_s^2, c_f^2,
+// 1/rho_t) is evaluated once
+template <int NDIM, int NDIR>
+__device__ __forceinline__ void cmax_all(const Pt<NDIR, double>& p, double gamma,
+                                         double (&cm)[NDIM]) {
+  const double rt = p.rho + p.rhob;
+  const double inv = 1.0 / rt;
+  double sm2 = 0.0, sb2 = 0.0, sbbg = 0.0, sbg2 = 0.0, sbt2 = 0.0;
+#pragma unroll
+  for (int c = 0; c < NDIR; c++) {
+    sm2 += p.m[c] * p.m[c];
+    sb2 += p.b[c] * p.b[c];
+    sbbg += p.b[c] * p.bg[c];
+    sbg2 += p.bg[c] * p.bg[c];
+    double bt = p.b[c] + p.bg[c];
+    sbt2 += bt * bt;
+  }
+  double pth = 0.5 * sm2 * inv;
+  pth = pth + 0.5 * sb2 + sbbg;
+  pth = (gamma - 1.0) * (p.e - pth);
+  const double cs2 = gamma * (pth + (gamma - 1.0) * (p.eb - 0.5 * sbg2)) * inv;
+  const double cf2 = sbt2 * inv + cs2;
+  const double cf4 = cf2 * cf2, cs4 = 4.0 * cs2 * inv;
+#pragma unroll
+  for (int d = 0; d < NDIM; d++) {
+    const double bd = p.b[d] + p.bg[d];
+    cm[d] = sqrt(0.5 * (cf2 + sqrt(cf4 - cs4 * (bd * bd)))) + fabs(p.m[d] * inv);
+  }
+}
+
+// ---------------------------------------------------------------------
+// block-wide max reduction into uint64 atomics (values are >= 0)
+
+template <int NV>
+__device__ __forceinline__ void block_max_atomic(double (&v)[NV], unsigned long long* dst) {
+  __shared__ double red[NV][16];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    if (lane == 0) red[k][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int nwave = (blockDim.x + 63) >> 6;
+    double x = 0.0;
+    for (int j = 0; j < nwave; j++) x = fmax(x, red[threadIdx.x][j]);
+    atomicMax(dst + threadIdx.x, (unsigned long long)__double_as_longlong(x));
+  }
+}
+
+__device__ __forceinline__ double red_get(const unsigned long long* p, int k) {
+  return __longlong_as_double((long long)p[k]);
+}
+
+// ---------------------------------------------------------------------
+// index helpers
+
+__device__ __forceinline__ void decode_box(long long t, const int (&lo)[3], const int (&ext)[3],
+                                           int (&c)[3]) {
+  c[0] = lo[0] + (int)(t % ext[0]);
+  t /= ext[0];
+  c[1] = lo[1] + (int)(t % ext[1]);
+  t /= ext[1];
+  c[2] = lo[2] + (int)t;
+}
+
+__device__ __forceinline__ long long lin(const GridDesc& g, int i0, int i1, int i2) {
+  return (long long)i0 * g.st[0] + (long long)i1 * g.st[1] + (long long)i2 * g.st[2];
+}
+
+// setnu ratio md3/md1 at the face between q[j] and q[j+1] of a 6-value
+// window q[j-2..j+3] (src/vacusr.viscosity.t:413-427)
+// The q values arrive rounded, as the reference stores them in tmp_nuI; no
+// product may be fused into these differences, or a constant region (all
+// q equal) would show rounding residues instead of exact zeros, and the
+// ratio there (and maxviscoef, and dt) would depend on them.
+template <typename C>
+__device__ __forceinline__ C face_ratio(const C* q) {
+#pragma clang fp contract(off)
+  C d1[5], d3[3];
+#pragma unroll
+  for (int j = 0; j < 5; j++) d1[j] = fabs(q[j + 1] - q[j]);
+#pragma unroll
+  for (int j = 0; j < 3; j++) d3[j] = fabs((C)3 * (q[j + 2] - q[j + 1]) - (q[j + 3] - q[j]));
+  const C md3 = fmax(fmax(d3[0], d3[1]), d3[2]);
+  const C md1 = fmax(fmax(fmax(d1[0], d1[1]), fmax(d1[2], d1[3])), d1[4]);
+  return (md1 > (C)0) ? cdiv(md3, md1) : (C)0;
+}
+
+__device__ __forceinline__ int cross_index(int ndim, int k, int l) {
+  return 2 * (k * (ndim - 1) + (l < k ? l : l - 1));
+}
+
+// Workgroups are dealt to the 8 XCDs of the device in turn; renumbering them
+// so that each XCD takes one contiguous range of cells keeps the neighbour
+// rows of a line stencil in the same XCD's L2.  The grid is a multiple of 8
+// blocks, so the renumbering is a bijection.
+__device__ __forceinline__ long long xcd_block(int xcd) {
+  const unsigned nb = gridDim.x;
+  if (xcd <= 1 || (nb % (unsigned)xcd) != 0) return blockIdx.x;
+  const unsigned per = nb / (unsigned)xcd;
+  return (long long)(blockIdx.x % (unsigned)xcd) * per + blockIdx.x / (unsigned)xcd;
+}
+
+// ---------------------------------------------------------------------
+// kernel 1: the work of one axis K for a group of CK consecutive mesh cells
+// along K, all of which lies on the K-line through the group:
+//   - 4th-order flux divergence along K (centdiff4, src/vaccd.t:35-70)
+//     and the background energy terms with derivatives along K,
+//   - setnu (src/vacusr.viscosity.t:224-455) for the 2*ndim+1 variables of
+//     axis K from the 7-point stencil (mirror layer beyond the grid),
+//   - the hyperdiffusion terms along K of addsource_visc (rho, thermal
+//     energy, momentum i==k branch, field m==k branch),
+//   - the cross-term factors T (diffusion along K) that kernel 2
+//     differentiates along the other axes:
+//       Tm_{K,l} = d_K v_l * (nuL+nuR)_{m_l}/4,  Tb_{K,l} = d_K b_l * (nuL+nuR)_{b_l}/2.
+// The group shares its setnu work: the q values of the CK+6 stencil points
+// and the CK+1 face ratios are evaluated once for the CK cells.  The
+// contribution is added to the rate accumulator R (written on K=0).
+
+template <int K>
+struct LineGroup {
+  static constexpr int CK = (K == 0) ? SAC_CK_X : SAC_CK_YZ;
+};
+
+// One group of CK consecutive mesh cells along K starting at c[K] (other
+// coordinates c[d]) of the grid g; adds the group's maximum nu to numax.
+template <int NDIM, int NDIR, int K, int CK, typename S, typename C>
+__device__ __forceinline__ void axis_group(const AxisArgs& a, const GridDesc& g, const int (&c)[3],
+                                           C& numax) {
+  constexpr int NE = 2 + 2 * NDIR;
+  constexpr int NP = 2 * NDIM + 1;
+  const int c0 = c[K];
+  const long long fs = g.fs;
+  const S* __restrict__ W = (const S*)a.wct;
+  const double* __restrict__ B = a.bg;
+  const double gamma = a.gamma;
+  const long long ic0 = lin(g, c[0], c[1], c[2]);
+  const long long sk = g.st[K];
+  const int GK = g.G[K], MloK = g.Mlo[K], MhiK = g.Mhi[K];
+  // per-cell geometry; cells beyond the mesh are computed on clamped
+  // indices and never read back
+  bool valid[CK], fluxc[CK], z1[CK];
+  C dxk[CK], idx[CK];
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    const int cj = c0 + j;
+    valid[j] = cj <= MhiK;
+    fluxc[j] = valid[j] && cj >= MloK + 2 && cj <= MhiK - 2;
+    z1[j] = (cj == MloK && !g.per_lo[K]) || (cj == MhiK && !g.per_hi[K]);
+    const double dxd = a.dx[K][cj < GK ? cj : GK - 1];
+    dxk[j] = (C)dxd;
+    idx[j] = (C)(1.0 / dxd);
+  }
+  C R[CK][NE];
+
+  // ---- flux divergence along K (zero on two layers at each side) ----
+  // cell by cell: the four neighbour fluxes of a cell (the neighbours are
+  // shared with the next cells through the caches).  Branch-free: a cell
+  // outside the flux range is evaluated on an in-grid cell and its result
+  // replaced by zero.
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    const int cf = (c0 + j > MhiK) ? MhiK : c0 + j;
+    const long long ic = ic0 + (long long)(cf - c0) * sk;
+    C bgk, bgl[NDIM], pTb0;
+    {
+      Pt<NDIR, C> p0;
+      load_pt<NDIR, S, C>(W, B, fs, ic, p0);
+      pTb0 = (C)ptotal_bg<NDIR, C>(p0, gamma);
+      bgk = (C)p0.bg[K];
+#pragma unroll
+      for (int l = 0; l < NDIM; l++) bgl[l] = (C)p0.bg[l];
+    }
+    const C inv12 = idx[j] / (C)12;
+    C Fm[NE], D1[NE], E1[NE], vm[NDIM], dv1[NDIM];
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      load_pt<NDIR, S, C>(W, B, fs, ic - sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, Fm, vm);
+    }
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      C F[NE], v[NDIM];
+      load_pt<NDIR, S, C>(W, B, fs, ic + sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, F, v);
+#pragma unroll
+      for (int n = 0; n < NE; n++) D1[n] = two_diff(F[n], Fm[n], E1[n]);
+#pragma unroll
+      for (int l = 0; l < NDIM; l++) dv1[l] = v[l] - vm[l];
+    }
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      load_pt<NDIR, S, C>(W, B, fs, ic - 2 * sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, Fm, vm);
+    }
+    SCHED_FENCE();
+    {
+      Pt<NDIR, C> p;
+      C F[NE], v[NDIM];
+      load_pt<NDIR, S, C>(W, B, fs, ic + 2 * sk, p);
+      flux_pt<NDIR, NDIM, C>(p, K, gamma, F, v);
+      // -gradient4 = ((F2-F-2) - 8 (F1-F-1)) / (12 dx), with the rounding
+      // errors of the two differences added back (zero in FP64)
+#pragma unroll
+      for (int n = 0; n < NE; n++) {
+        C e2;
+        const C D2 = two_diff(F[n], Fm[n], e2);
+        R[j][n] = ((D2 - (C)8 * D1[n]) + (e2 - (C)8 * E1[n])) * inv12;
+      }
+      C re = 0;
+#pragma unroll
+      for (int l = 0; l < NDIM; l++) {
+        const C Dv = -((v[l] - vm[l]) - (C)8 * dv1[l]) * inv12;
+        if (l == K) re -= Dv * pTb0;
+        re += bgk * bgl[l] * Dv;
+      }
+      R[j][1 + NDIR] += re;
+#pragma unroll
+      for (int n = 0; n < NE; n++) R[j][n] = fluxc[j] ? R[j][n] : (C)0;
+    }
+    SCHED_FENCE();
+  }
+
+  // ---- hyperdiffusion along K ----
+  if (a.visc_on) {
+    const C ctk = (C)red_get(a.ctot_src, K);
+    constexpr int NQ = CK + 6;  // stencil points at offsets -3..CK+2
+    int off[NQ];
+    double inv7[NQ];
+#pragma unroll
+    for (int s = 0; s < NQ; s++) {
+      int j = c0 + s - 3;
+      if (j < 0) j = MloK + 3;       // mirror layer below the grid
+      if (j > GK - 1) j = MhiK - 3;  // and above it
+      off[s] = (j - c0) * (int)sk;
+      const long long i = ic0 + off[s];
+      inv7[s] = recip<C>((double)W[i] + B[fs + i]);
+    }
+    // rho_t of the cells and their direct neighbours (offsets -1..CK)
+    C rtw[CK + 2];
+    int offn[CK + 2];
+#pragma unroll
+    for (int s = 0; s < CK + 2; s++) {
+      int j = c0 + s - 1;
+      j = j > GK - 1 ? GK - 1 : j;
+      offn[s] = (j - c0) * (int)sk;
+      const long long i = ic0 + offn[s];
+      rtw[s] = (C)((double)W[i] + B[fs + i]);
+    }
+    // p = 0: rho' ; p = 1: thermal energy ; p = 2+l: v_l ; then b_l (l != K).
+    // No FP contraction here: differences of q (face ratios, gradient1L/R)
+    // must see the rounded q values, as in the reference.
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+#pragma clang fp contract(off)
+      C q[NQ];
+      int lb = -1;
+      if (p >= 2 + NDIM) {
+        int n = p - 2 - NDIM;
+        lb = (n < K) ? n : n + 1;
+      }
+#pragma unroll
+      for (int s = 0; s < NQ; s++) {
+        const long long i = ic0 + off[s];
+        if (p == 0) {
+          q[s] = (C)W[i];
+        } else if (p == 1) {
+          // thermal energy e - (b^2 + m^2/rho_t)/2, recovered in FP64
+          double sb = 0.0, sm = 0.0;
+#pragma unroll
+          for (int cd = 0; cd < NDIR; cd++) {
+            const double mc = (double)W[(1 + cd) * fs + i];
+            const double bc = (double)W[(2 + NDIR + cd) * fs + i];
+            sb += bc * bc;
+            sm += mc * mc;
+          }
+          q[s] = (C)((double)W[(1 + NDIR) * fs + i] - 0.5 * (sb + sm * inv7[s]));
+        } else if (p < 2 + NDIM) {
+          q[s] = (C)W[(p - 1) * fs + i] * (C)inv7[s];
+        } else {
+          q[s] = (C)W[(2 + NDIR + lb) * fs + i];
+        }
+      }
+      const C ch = (p >= 2 && p < 2 + NDIM) ? (C)0.4 : (C)0.04;
+      const C Kc = ctk * ch;
+      // face f (between offsets f-1 and f) from the window q[f..f+5]
+      C ratio[CK + 1];
+#pragma unroll
+      for (int f = 0; f < CK + 1; f++) ratio[f] = face_ratio<C>(q + f);
+#pragma unroll
+      for (int j = 0; j < CK; j++) {
+        const C nuR = Kc * ratio[j + 1] * dxk[j];
+        const C nuL = Kc * ratio[j] * dxk[j];
+        numax = valid[j] ? fmax(numax, fmax(nuR, nuL)) : numax;
+        const C qm = q[j + 2], q0 = q[j + 3], qp = q[j + 4];
+        const C gL = z1[j] ? (C)0 : (q0 - qm) * idx[j];
+        const C gR = z1[j] ? (C)0 : (qp - q0) * idx[j];
+        const C tL = nuL * gL, tR = nuR * gR;
+        const long long icj = ic0 + offn[j + 1];
+        if (p == 0) {
+          R[j][0] += (tR - tL) * idx[j];
+        } else if (p == 1) {
+          R[j][1 + NDIR] += (tR - tL) * idx[j];
+        } else if (p < 2 + NDIM) {
+          const int l = p - 2;
+          const C rhoL = (rtw[j + 1] + rtw[j]) * (C)0.5, rhoR = (rtw[j + 2] + rtw[j + 1]) * (C)0.5;
+          const S* __restrict__ Wm = W + (1 + l) * fs + ic0;
+          const C m0 = (C)Wm[offn[j + 1]];
+          const C mL = (m0 + (C)Wm[offn[j]]) * (C)0.5;
+          const C mR = ((C)Wm[offn[j + 2]] + m0) * (C)0.5;
+          const C mult = (l == K) ? (C)1 : (C)0.5;  // the i==k branch runs twice for l==k
+          R[j][1 + l] += mult * (rhoR * tR - rhoL * tL) * idx[j];
+          R[j][1 + NDIR] += mult * (mR * tR - mL * tL) * idx[j];
+          if (l != K) {
+            const C gC = z1[j] ? (C)0 : (qp - qm) * idx[j] * (C)0.5;
+            ((S*)a.T)[(long long)cross_index(NDIM, K, l) * fs + icj] =
+                (S)(gC * (nuL + nuR) * (C)0.25);
+          }
+        } else {
+          const C BL = (q0 + qm) * (C)0.5, BR = (qp + q0) * (C)0.5;
+          R[j][2 + NDIR + lb] += (tR - tL) * idx[j];
+          R[j][1 + NDIR] += (tR * BR - tL * BL) * idx[j];
+          const C gC = z1[j] ? (C)0 : (qp - qm) * idx[j] * (C)0.5;
+          ((S*)a.T)[(long long)(cross_index(NDIM, K, lb) + 1) * fs + icj] =
+              (S)(gC * (nuL + nuR) * (C)0.5);
+        }
+      }
+      SCHED_FENCE();
+    }
+  }
+  // ---- accumulate ----
+  // Stores are unconditional (branches here cost registers): a cell past
+  // the mesh end writes R and T of a ghost cell (index clamped to the
+  // grid), which nothing reads.
+  S* __restrict__ Racc = (S*)a.R;
+#pragma unroll
+  for (int j = 0; j < CK; j++) {
+    const int cj = (c0 + j > GK - 1) ? GK - 1 : c0 + j;
+    const long long icj = ic0 + (long long)(cj - c0) * sk;
+    if (K == 0) {
+#pragma unroll
+      for (int n = 0; n < NE; n++) Racc[n * fs + icj] = (S)R[j][n];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NE; n++) Racc[n * fs + icj] = (S)((C)Racc[n * fs + icj] + R[j][n]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// kernel 2: gravity, derivatives of the cross-term factors, the register
+// update out = base + qa*R (acc_out = acc_in + qb*R) and the cmax
+// reductions of the result.
+//   m_k += d_l(rho_t Tm_{k,l}),  e += d_l(m_k Tm_{k,l})
+//   b_k -= d_l(Tb_{k,l}),        e -= d_l(b_k Tb_{k,l})
+// with central differences along l, zero at a non-periodic edge of l, and
+// T = 0 outside the mesh (nu is never set there, src/vacusr.viscosity.t:116-129).
+// The register update runs in FP64 and is rounded once to S.
+
+// One mesh cell c of the grid g; the cmax values of the result go to redv.
+template <int NDIM, int NDIR, typename S, typename C>
+__device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g, const int (&c)[3],
+                                           double (&redv)[2 * NDIM]) {
+  constexpr int NE = 2 + 2 * NDIR;
+  const long long fs = g.fs;
+  const S* __restrict__ W = (const S*)a.wct;
+  const S* __restrict__ RR = (const S*)a.R;
+  const S* __restrict__ TT = (const S*)a.T;
+  const double* __restrict__ B = a.bg;
+  const long long ic = lin(g, c[0], c[1], c[2]);
+  C R[NE];
+#pragma unroll
+  for (int n = 0; n < NE; n++) R[n] = (C)RR[n * fs + ic];
+  // gravity on the perturbed density (addsource_grav, src/vacusr.gravity.t:46-78)
+  {
+    const C rho = (C)W[ic];
+    const C inv = (C)recip<C>((double)W[ic] + B[fs + ic]);
+    C rg = 0;
+#pragma unroll
+    for (int d = 0; d < NDIM; d++) {
+      if (a.grav_on[d]) {
+        const C gd = (C)a.grav[d];
+        R[1 + d] += gd * rho;
+        rg += gd * rho * (C)W[(1 + d) * fs + ic] * inv;
+      }
+    }
+    R[1 + NDIR] += rg;
+  }
+  if (a.visc_on && NDIM > 1) {
+#pragma unroll
+    for (int l = 0; l < NDIM; l++) {
+      if ((c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l])) continue;
+      const long long sl = g.st[l];
+      const C h = (C)(0.5 / a.dx[l][c[l]]);
+      const bool inP = c[l] + 1 <= g.Mhi[l], inM = c[l] - 1 >= g.Mlo[l];
+      const long long ip = ic + sl, im = ic - sl;
+      const C rtp = (C)((double)W[ip] + B[fs + ip]), rtm = (C)((double)W[im] + B[fs + im]);
+#pragma unroll
+      for (int k = 0; k < NDIM; k++) {
+        if (k == l) continue;
+        const long long xi = (long long)cross_index(NDIM, k, l) * fs;
+        const C Tmp = inP ? (C)TT[xi + ip] : (C)0, Tmm = inM ? (C)TT[xi + im] : (C)0;
+        const C Tbp = inP ? (C)TT[xi + fs + ip] : (C)0, Tbm = inM ? (C)TT[xi + fs + im] : (C)0;
+        const C mkp = (C)W[(1 + k) * fs + ip], mkm = (C)W[(1 + k) * fs + im];
+        const C bkp = (C)W[(2 + NDIR + k) * fs + ip], bkm = (C)W[(2 + NDIR + k) * fs + im];
+        R[1 + k] += (rtp * Tmp - rtm * Tmm) * h;
+        R[1 + NDIR] += (mkp * Tmp - mkm * Tmm) * h;
+        R[2 + NDIR + k] -= (Tbp - Tbm) * h;
+        R[1 + NDIR] -= (Tbp * bkp - Tbm * bkm) * h;
+      }
+    }
+  }
+  // register update and cmax of the (stored) result
+  Pt<NDIR, double> po;
+  {
+    S* __restrict__ out = (S*)a.out;
+    const S* __restrict__ base = (const S*)a.base;
+    double vals[NE];
+#pragma unroll
+    for (int n = 0; n < NE; n++) {
+      const S o = (S)((double)base[n * fs + ic] + a.qa * (double)R[n]);
+      out[n * fs + ic] = o;
+      vals[n] = (double)o;
+    }
+    if (a.acc_out) {
+      S* __restrict__ ao = (S*)a.acc_out;
+      const S* __restrict__ ai = (const S*)a.acc_in;
+#pragma unroll
+      for (int n = 0; n < NE; n++)
+        ao[n * fs + ic] = (S)((double)ai[n * fs + ic] + a.qb * (double)R[n]);
+    }
+    po.rho = vals[0];
+#pragma unroll
+    for (int cd = 0; cd < NDIR; cd++) {
+      po.m[cd] = vals[1 + cd];
+      po.b[cd] = vals[2 + NDIR + cd];
+      po.bg[cd] = B[(2 + cd) * fs + ic];
+    }
+    po.e = vals[1 + NDIR];
+    po.eb = B[ic];
+    po.rhob = B[fs + ic];
+  }
+  double cm[NDIM];
+  cmax_all<NDIM, NDIR>(po, a.gamma, cm);
+#pragma unroll
+  for (int d = 0; d < NDIM; d++) {
+    redv[d] = cm[d];
+    redv[NDIM + d] = cm[d] / a.dx[d][c[d]];
+  }
+}
+
+// ---------------------------------------------------------------------
+// kernel 4: ghost-cell fill.  The reference fills boundary regions one
+// after another (iB = 1..2*ndim), each spanning the whole grid along the
+// other axes, so a cell that is ghost along several axes ends up with the
+// value written by its highest such axis, taken from a cell that was
+// filled earlier.  Following that chain backwards reaches a mesh cell (or
+// the stored value of a 'fixed' region), which is what every thread does
+// here; no thread reads a ghost cell, so the kernel is race-free.
+
+template <typename S>
+__device__ __forceinline__ S ghost_value(const GhostArgs& a, int var, int c0, int c1, int c2) {
+  const GridDesc& g = a.g;
+  const S* f = (const S*)a.f;
+  const S* snap = (const S*)a.snap;
+  int c[3] = {c0, c1, c2};
+  bool neg = false;
+  for (int iter = 0; iter < 8; iter++) {
+    int d = -1;
+#pragma unroll
+    for (int dd = 2; dd >= 0; dd--) {
+      if (d < 0 && dd < g.ndim && (c[dd] < g.Mlo[dd] || c[dd] > g.Mhi[dd])) d = dd;
+    }
+    const long long i = lin(g, c[0], c[1], c[2]);
+    if (d < 0) return neg ? -f[(long long)var * g.fs + i] : f[(long long)var * g.fs + i];
+    const int side = (c[d] > g.Mhi[d]) ? 1 : 0;
+    const int code = a.bc[var * 6 + 2 * d + side];
+    const int N = g.Mhi[d] - g.Mlo[d] + 1;
+    if (code == BC_PERIODIC) {
+      c[d] += side ? -N : N;
+    } else if (code == BC_FIXED && snap) {
+      return neg ? -snap[(long long)var * g.fs + i] : snap[(long long)var * g.fs + i];
+    } else if (code == BC_FIXED && a.keep_fixed) {
+      return neg ? -f[(long long)var * g.fs + i] : f[(long long)var * g.fs + i];
+    } else if (code == BC_CONT || code == BC_FIXED) {
+      c[d] = side ? g.Mhi[d] : g.Mlo[d];
+    } else {  // symm, symm0, asymm: reflection about the boundary face
+      c[d] = side ? (2 * g.Mhi[d] + 1 - c[d]) : (2 * g.Mlo[d] - 1 - c[d]);
+      if (code == BC_ASYMM) neg = !neg;
+    }
+  }
+  return (S)0;
+}
+
+}  // namespace sac

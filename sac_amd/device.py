@@ -238,17 +238,22 @@ class Device:
         return b.value
 
 
-def run(prob, steps=None, tmax=None, typeadvance="fourstep"):
-    """Upload, getboundary, then the main loop until itmax/tmax; returns the
-    final state and the list of dt."""
+def run(prob, steps=None, tmax=None, typeadvance="fourstep", dtmin=1.0e-99):
+    """Upload, getboundary, then the main loop (src/vac.t:55-106) until
+    itmax, tmax or dt < dtmin (timetofinish, src/vac.t:585-605); returns the
+    final state and the list of dt of the steps taken."""
     dev = Device(prob, typeadvance=typeadvance)
     dev.upload(prob.w)
     dev.getboundary()
     t, it, dts = 0.0, 0, []
     steps = prob.itmax if steps is None else steps
     tmax = prob.tmax if tmax is None else tmax
-    while it < steps and t < tmax:
-        dt = dev.step(prob.courantpar)
+    while True:
+        dt, _ = dev.getdt_courant(prob.courantpar)
+        dt = dev.getdt_special(dt)
+        if it >= steps or t >= tmax or dt < dtmin:
+            break
+        dev.advance(dt)
         dts.append(dt)
         t += dt
         it += 1

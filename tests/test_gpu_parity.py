@@ -145,13 +145,19 @@ def test_briowu_steps(dev_mod, oracle_mod):
     assert_close_state(prob, got, ref, TOL, 6, "brio-wu")
 
 
-@pytest.mark.parametrize("case", ["briowu", "strat3d"])
+@pytest.mark.parametrize("case", ["briowu", "strat3d", "tube_constant_regions"])
 def test_stage_maxviscoef_and_dt(dev_mod, oracle_mod, case):
     """maxviscoef gathered by one stage, and the courant dt, on identical
-    inputs"""
+    inputs.  The tube case has exactly constant regions with nonzero
+    velocity: setnu's ratio there must come out as exact zeros (no rounding
+    residue from a product fused into the q differences)."""
     if case == "briowu":
         prob = problems.brio_wu(n=400)
         nsteps = 3
+    elif case == "tube_constant_regions":
+        n, left, right = problems.tube_ensemble(count=1, nmin=117, nmax=117, seed=5)[0]
+        prob = problems.riemann_tube(n, left, right)
+        nsteps = 0
     else:
         prob = problems.stratified(n=12, nxyz=(12, 10, 8))
         nsteps = 2
