@@ -63,6 +63,7 @@ extern "C" {
 #define SAC_K_AXIS1 5 /* k_axis launches of axis 1, 2, 3 separately */
 #define SAC_K_AXIS2 6
 #define SAC_K_AXIS3 7
+#define SAC_K_CLOSURE 8 /* learned-closure inference launches (one per axis) */
 
 typedef void *sac_handle;
 
@@ -152,6 +153,42 @@ int sac_dev_memory(sac_handle h, long long *bytes);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *sac_dev_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Learned hyperdiffusion closure (FP64 handles).  A per-face MLP
+ * (21 -> 64 -> 64 -> 64 -> 7, BF16 matrix-core arithmetic with FP32
+ * accumulation) predicts setnu's ratio md3/md1 (src/vacusr.viscosity.t:
+ * 413-427) of every variable of an axis from a 7-point stencil of rho',
+ * v_K and a transverse b, normalised by the local density and sound speed.
+ * Faces where the shock sensor (largest normalised jump of rho' or v_K over
+ * the stencil) reaches the threshold, or near the grid edge, keep the exact
+ * ratio.  No reference counterpart: it replaces the coefficient computation
+ * inside setnu when enabled; everything else of the step is unchanged.
+ * ---------------------------------------------------------------------- */
+int sac_closure_nparam(void);
+/* Parameters (He-normal from seed), Adam state, room for `capacity`
+ * training samples and per-block gradients for up to max_blocks wavefronts. */
+int sac_closure_init(sac_handle h, long long capacity, int max_blocks, unsigned seed);
+int sac_closure_set_params(sac_handle h, const float *p, int n);
+int sac_closure_get_params(sac_handle h, float *p, int n);
+/* Append (stencil, exact ratio) samples of every mesh face of every axis of
+ * register `slot`; faces past the sensor threshold get weight 0. */
+int sac_closure_collect(sac_handle h, int slot, long long *total);
+int sac_closure_clear(sac_handle h);
+/* `steps` Adam steps of blocks*64 samples each; sample k of the run is
+ * ((base + k) * stride) mod n (stride coprime with n walks a permutation).
+ * mean_loss: mean squared error of the last step. */
+int sac_closure_train(sac_handle h, int steps, int blocks, double lr, long long base,
+                      long long stride, double *mean_loss);
+/* Loss and parameter gradient of one forward/backward pass over the given
+ * samples (features [n][32] FP32, rounded to BF16; targets [n][8]: 7 ratios/6
+ * (<0: none) and a weight); n a multiple of 64. */
+int sac_closure_grad(sac_handle h, const float *feat, const float *targ, int nsamples,
+                     float *grad, double *loss);
+/* Use the learned ratios in advance/advect1 (on=1), stop using them (0), or
+ * (2) route the exact ratios through the same storage, a check of the
+ * learned path's indexing that must follow the exact run to FP32 rounding. */
+int sac_closure_enable(sac_handle h, int on, double sensor);
 
 /* ------------------------------------------------------------------------
  * Ensembles of independent 1-D tubes (ndim=1, ndir=3, nw=13, 'cont'

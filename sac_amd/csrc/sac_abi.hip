@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/sac_amd.h"
+#include "sac_closure.h"
 #include "sac_kernels.h"
 
 namespace {
@@ -38,6 +39,24 @@ int fail(int code, const char* fmt, ...) {
 constexpr int kSlots = 4;
 constexpr double kSmallDouble = 1.0e-99;  // src/vacdef.t:50
 constexpr double kBigDouble = 1.0e99;
+
+// learned-closure state of a handle (sac_closure_*)
+struct Closure {
+  float* P32 = nullptr;      // FP32 master parameters
+  uint16_t* P16 = nullptr;   // BF16 working copy
+  float* m1 = nullptr;       // Adam moments
+  float* m2 = nullptr;
+  float* partial = nullptr;  // per-block gradients [max_blocks][NPARAM]
+  float* loss = nullptr;     // per-block loss sums
+  int max_blocks = 0;
+  float* RF[3] = {nullptr, nullptr, nullptr};  // learned face ratios per axis
+  uint16_t* feat = nullptr;  // training samples
+  float* targ = nullptr;
+  long long cap = 0, n = 0;
+  int adam_t = 0;
+  int on = 0;
+  float sensor = 0.1f;
+};
 
 struct TimedLaunch {
   int kernel;
@@ -80,9 +99,10 @@ struct Dev {
   long long bytes = 0;
   bool timing = false;
   std::vector<TimedLaunch> timed;
-  double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  long long t_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double t_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long long t_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int xcd = 8;  // XCDs of the device (block renumbering of the line kernels)
+  Closure* cl = nullptr;
 };
 
 long long fieldsz(const Dev& d) { return d.g.fs; }
@@ -183,11 +203,22 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   x.visc_on = d.visc_on;
   x.ctot_src = red_of(d, wct);
   x.red_visc = red_visc(d);
+  const bool learned = d.cl && d.cl->on;
   for (int k = 0; k < d.ndim; k++) {
     // axis 1 (rows of a plane): neighbour rows in one XCD's L2.  Axes 0 and 2
     // keep the hardware order, whose round-robin already places the
     // neighbour planes of a row in the same XCD.
     x.xcd = (k == 1) ? d.xcd : 1;
+    x.RF = nullptr;
+    if (learned) {
+      hipEvent_t ei{};
+      t_begin(d, SAC_K_CLOSURE, &ei);
+      sac::closure::launch_infer(d.ndim, d.ndir, k, d.g, (const double*)d.reg[wct], d.bg, d.gamma,
+                                 d.cl->sensor, d.cl->P16, d.cl->P32, d.cl->RF[k], d.cl->on,
+                                 d.stream);
+      t_end(d, SAC_K_CLOSURE, ei);
+      x.RF = d.cl->RF[k];
+    }
     hipEvent_t e0{};
     t_begin(d, SAC_K_AXIS, &e0);
     sac::launch_axis(x, k, d.stream);
@@ -481,6 +512,19 @@ int sac_dev_free(sac_handle h) {
   hipFree(d->bc_e);
   hipFree(d->bc_b);
   hipFree(d->red);
+  if (d->cl) {
+    Closure* c = d->cl;
+    hipFree(c->P32);
+    hipFree(c->P16);
+    hipFree(c->m1);
+    hipFree(c->m2);
+    hipFree(c->partial);
+    hipFree(c->loss);
+    for (int k = 0; k < 3; k++) hipFree(c->RF[k]);
+    hipFree(c->feat);
+    hipFree(c->targ);
+    delete c;
+  }
   if (d->stream) hipStreamDestroy(d->stream);
   delete d;
   return 0;
@@ -670,7 +714,7 @@ int sac_dev_timing(sac_handle h, int enable) {
   Dev& d = *(Dev*)h;
   hipStreamSynchronize(d.stream);
   t_collect(d);
-  for (int k = 0; k < 8; k++) {
+  for (int k = 0; k < 9; k++) {
     d.t_ms[k] = 0.0;
     d.t_n[k] = 0;
   }
@@ -680,7 +724,7 @@ int sac_dev_timing(sac_handle h, int enable) {
 
 int sac_dev_timing_get(sac_handle h, int kernel, double* total_ms, long long* count) {
   if (int rc = check_handle(h)) return rc;
-  if (kernel < 0 || kernel > 7) return fail(SAC_E_ARG, "kernel id %d", kernel);
+  if (kernel < 0 || kernel > 8) return fail(SAC_E_ARG, "kernel id %d", kernel);
   Dev& d = *(Dev*)h;
   t_collect(d);
   if (total_ms) *total_ms = d.t_ms[kernel];
@@ -699,6 +743,215 @@ int sac_dev_synchronize(sac_handle h) {
 int sac_dev_memory(sac_handle h, long long* bytes) {
   if (int rc = check_handle(h)) return rc;
   if (bytes) *bytes = ((Dev*)h)->bytes;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * learned hyperdiffusion closure */
+
+int sac_closure_nparam(void) { return sac::mlp::NPARAM; }
+
+int sac_closure_init(sac_handle h, long long capacity, int max_blocks, unsigned seed) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (d.cl) return fail(SAC_E_ARG, "closure already initialised");
+  if (d.prec != SAC_PREC_FP64) return fail(SAC_E_UNSUPPORTED, "the learned closure runs in FP64 storage");
+  if (capacity < 0 || max_blocks < 1) return fail(SAC_E_ARG, "capacity/max_blocks");
+  Closure* c = new Closure();
+  d.cl = c;
+  const int NP = sac::mlp::NPARAM;
+  int rc = 0;
+  if ((rc = balloc(d, (void**)&c->P32, sizeof(float) * NP)) ||
+      (rc = balloc(d, (void**)&c->P16, sizeof(uint16_t) * NP)) ||
+      (rc = balloc(d, (void**)&c->m1, sizeof(float) * NP)) ||
+      (rc = balloc(d, (void**)&c->m2, sizeof(float) * NP)) ||
+      (rc = balloc(d, (void**)&c->partial, sizeof(float) * (size_t)NP * max_blocks)) ||
+      (rc = balloc(d, (void**)&c->loss, sizeof(float) * max_blocks)))
+    return rc;
+  c->max_blocks = max_blocks;
+  const int np_var = 2 * d.ndim + 1;
+  for (int k = 0; k < d.ndim; k++)
+    if ((rc = balloc(d, (void**)&c->RF[k], sizeof(float) * (size_t)np_var * d.g.fs))) return rc;
+  if (capacity > 0) {
+    if ((rc = balloc(d, (void**)&c->feat, sizeof(uint16_t) * sac::closure::kFeat * (size_t)capacity)) ||
+        (rc = balloc(d, (void**)&c->targ, sizeof(float) * sac::closure::kTarg * (size_t)capacity)))
+      return rc;
+  }
+  c->cap = capacity;
+  // He-normal initial weights (fan-in), zero biases, from a seeded
+  // xorshift + Box-Muller stream
+  std::vector<float> p(NP, 0.f);
+  unsigned long long st = 0x9E3779B97F4A7C15ull ^ (unsigned long long)seed;
+  auto uni = [&]() {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    return ((st >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int off, int rows, int cols) {
+    const double sd = std::sqrt(2.0 / cols);
+    for (int i = 0; i < rows * cols; i++) {
+      const double u1 = uni(), u2 = uni();
+      p[off + i] = (float)(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+    }
+  };
+  using namespace sac::mlp;
+  fill(OFF_W1, H, IN);
+  fill(OFF_W2, H, H);
+  fill(OFF_W3, H, H);
+  fill(OFF_W4, OUT, H);
+  HIPCHK(hipMemcpy(c->P32, p.data(), sizeof(float) * NP, hipMemcpyHostToDevice));
+  sac::closure::launch_to_bf16(c->P32, c->P16, d.stream);
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return 0;
+}
+
+static int closure_of(sac_handle h, Closure** out) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (!d.cl) return fail(SAC_E_ARG, "sac_closure_init has not run on this handle");
+  *out = d.cl;
+  return 0;
+}
+
+int sac_closure_set_params(sac_handle h, const float* p, int n) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  if (!p || n != sac::mlp::NPARAM) return fail(SAC_E_ARG, "need %d parameters", sac::mlp::NPARAM);
+  Dev& d = *(Dev*)h;
+  HIPCHK(hipMemcpy(c->P32, p, sizeof(float) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->m1, 0, sizeof(float) * n));
+  HIPCHK(hipMemset(c->m2, 0, sizeof(float) * n));
+  c->adam_t = 0;
+  sac::closure::launch_to_bf16(c->P32, c->P16, d.stream);
+  HIPCHK(hipStreamSynchronize(d.stream));
+  return 0;
+}
+
+int sac_closure_get_params(sac_handle h, float* p, int n) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  if (!p || n != sac::mlp::NPARAM) return fail(SAC_E_ARG, "need %d parameters", sac::mlp::NPARAM);
+  HIPCHK(hipMemcpy(p, c->P32, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int sac_closure_collect(sac_handle h, int slot, long long* total) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  Dev& d = *(Dev*)h;
+  if (slot < 0 || slot >= kSlots) return fail(SAC_E_ARG, "bad slot %d", slot);
+  long long nm = 1;
+  for (int k = 0; k < 3; k++) nm *= d.g.Mhi[k] - d.g.Mlo[k] + 1;
+  const long long add = nm * d.ndim;
+  if (c->n + add > c->cap)
+    return fail(SAC_E_ARG, "sample buffer full (%lld + %lld > %lld)", c->n, add, c->cap);
+  sac::closure::launch_collect(d.ndim, d.ndir, d.g, (const double*)d.reg[slot], d.bg, d.gamma,
+                               c->sensor, c->feat, c->targ, c->n, d.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  c->n += add;
+  if (total) *total = c->n;
+  return 0;
+}
+
+int sac_closure_clear(sac_handle h) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  c->n = 0;
+  return 0;
+}
+
+int sac_closure_train(sac_handle h, int steps, int blocks, double lr, long long base,
+                      long long stride, double* mean_loss) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  Dev& d = *(Dev*)h;
+  if (c->n < 64) return fail(SAC_E_ARG, "no training samples");
+  if (blocks < 1 || blocks > c->max_blocks) return fail(SAC_E_ARG, "blocks %d", blocks);
+  if (stride < 1) return fail(SAC_E_ARG, "stride");
+  const long long per = (long long)blocks * sac::mlp::WAVE;
+  double lsum = 0.0;
+  std::vector<float> lb(blocks);
+  for (int s = 0; s < steps; s++) {
+    sac::closure::launch_train(c->feat, c->targ, c->n, base + (long long)s * per, stride, c->P16,
+                               c->P32, c->partial, c->loss, blocks, 1.0f / (float)per, d.stream);
+    c->adam_t += 1;
+    sac::closure::launch_adam(c->P32, c->P16, c->m1, c->m2, c->partial, blocks, (float)lr, 0.9f,
+                              0.999f, 1e-8f, c->adam_t, d.stream);
+    if (mean_loss && s == steps - 1) {
+      HIPCHK(hipMemcpyAsync(lb.data(), c->loss, sizeof(float) * blocks, hipMemcpyDeviceToHost,
+                            d.stream));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  if (mean_loss) {
+    for (int b = 0; b < blocks; b++) lsum += lb[b];
+    *mean_loss = lsum / (double)per;
+  }
+  return 0;
+}
+
+int sac_closure_grad(sac_handle h, const float* feat, const float* targ, int nsamples,
+                     float* grad, double* loss) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  Dev& d = *(Dev*)h;
+  if (!feat || !targ || !grad || nsamples < 64 || nsamples % 64 != 0)
+    return fail(SAC_E_ARG, "nsamples must be a positive multiple of 64");
+  const int blocks = nsamples / 64;
+  if (blocks > c->max_blocks) return fail(SAC_E_ARG, "too many samples");
+  const int NP = sac::mlp::NPARAM;
+  uint16_t* f16 = nullptr;
+  float* tg = nullptr;
+  HIPCHK(hipMalloc((void**)&f16, sizeof(uint16_t) * sac::closure::kFeat * nsamples));
+  HIPCHK(hipMalloc((void**)&tg, sizeof(float) * sac::closure::kTarg * nsamples));
+  // features to BF16 with round-to-nearest-even, as the sample kernel does
+  std::vector<uint16_t> hb((size_t)sac::closure::kFeat * nsamples);
+  for (size_t i = 0; i < hb.size(); i++) {
+    uint32_t u;
+    memcpy(&u, &feat[i], 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    hb[i] = (uint16_t)(u >> 16);
+  }
+  int rc = 0;
+  if (hipMemcpy(f16, hb.data(), sizeof(uint16_t) * hb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(tg, targ, sizeof(float) * sac::closure::kTarg * nsamples, hipMemcpyHostToDevice) !=
+          hipSuccess)
+    rc = fail(SAC_E_HIP, "upload");
+  if (!rc) {
+    sac::closure::launch_train(f16, tg, nsamples, 0, 1, c->P16, c->P32, c->partial, c->loss,
+                               blocks, 1.0f / (float)nsamples, d.stream);
+    std::vector<float> part((size_t)NP * blocks), lb(blocks);
+    if (hipStreamSynchronize(d.stream) != hipSuccess ||
+        hipMemcpy(part.data(), c->partial, sizeof(float) * part.size(), hipMemcpyDeviceToHost) !=
+            hipSuccess ||
+        hipMemcpy(lb.data(), c->loss, sizeof(float) * blocks, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(SAC_E_HIP, "training kernel: %s", hipGetErrorString(hipGetLastError()));
+    } else {
+      for (int k = 0; k < NP; k++) {
+        double g = 0.0;
+        for (int b = 0; b < blocks; b++) g += part[(size_t)b * NP + k];
+        grad[k] = (float)g;
+      }
+      double ls = 0.0;
+      for (int b = 0; b < blocks; b++) ls += lb[b];
+      if (loss) *loss = ls / nsamples;
+    }
+  }
+  hipFree(f16);
+  hipFree(tg);
+  return rc;
+}
+
+int sac_closure_enable(sac_handle h, int on, double sensor) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  if (!(sensor > 0.0)) return fail(SAC_E_ARG, "sensor threshold must be > 0");
+  if (on < 0 || on > 2) return fail(SAC_E_ARG, "on must be 0, 1 or 2");
+  c->on = on;
+  c->sensor = (float)sensor;
   return 0;
 }
 

@@ -300,7 +300,7 @@ struct LineGroup {
 
 // One group of CK consecutive mesh cells along K starting at c[K] (other
 // coordinates c[d]) of the grid g; adds the group's maximum nu to numax.
-template <int NDIM, int NDIR, int K, int CK, typename S, typename C>
+template <int NDIM, int NDIR, int K, int CK, typename S, typename C, bool LEARNED = false>
 __device__ __forceinline__ void axis_group(const AxisArgs& a, const GridDesc& g, const int (&c)[3],
                                            C& numax) {
   constexpr int NE = 2 + 2 * NDIR;
@@ -465,7 +465,19 @@ __device__ __forceinline__ void axis_group(const AxisArgs& a, const GridDesc& g,
       // face f (between offsets f-1 and f) from the window q[f..f+5]
       C ratio[CK + 1];
 #pragma unroll
-      for (int f = 0; f < CK + 1; f++) ratio[f] = face_ratio<C>(q + f);
+      for (int f = 0; f < CK + 1; f++) {
+        // learned closure: the predicted ratio of this face (stored at its
+        // left cell), or -1 where the exact ratio is required
+        if constexpr (LEARNED) {
+          const float rf = a.RF[(long long)p * fs + ic0 + (long long)(f - 1) * sk];
+          if (rf >= 0.f)
+            ratio[f] = (C)rf;
+          else
+            ratio[f] = face_ratio<C>(q + f);
+        } else {
+          ratio[f] = face_ratio<C>(q + f);
+        }
+      }
 #pragma unroll
       for (int j = 0; j < CK; j++) {
         const C nuR = Kc * ratio[j + 1] * dxk[j];

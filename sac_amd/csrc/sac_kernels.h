@@ -20,6 +20,7 @@ struct AxisArgs {
   const unsigned long long* ctot_src;  // c_tot per direction of wct
   unsigned long long* red_visc;        // maxviscoef accumulator
   int xcd;                             // XCD count for block renumbering (1 = off)
+  const float* RF;                     // learned face ratios of axis K, or null (exact)
 };
 
 struct FinalArgs {

@@ -28,7 +28,7 @@
 namespace sac {
 
 // kernel 1: line groups of axis K (see axis_group)
-template <int NDIM, int NDIR, int K, typename S, typename C>
+template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 __global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
   constexpr int CK = LineGroup<K>::CK;
   const GridDesc& g = a.g;
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) c[d] = lo[d] + ((d == K) ? c[d] * CK : c[d]);
-    axis_group<NDIM, NDIR, K, CK, S, C>(a, g, c, numax);
+    axis_group<NDIM, NDIR, K, CK, S, C, LEARNED>(a, g, c, numax);
   }
   if (a.visc_on) {
     double r2[1] = {(double)numax};
@@ -207,6 +207,19 @@ static void launch_nd(int which, int axis, const void* args, hipStream_t st, lon
   if (nb == 0) return;
   if (which == 0) {
     const AxisArgs& A = *(const AxisArgs*)args;
+    if (A.RF) {  // learned closure (FP64 storage only)
+      if constexpr (sizeof(S) == sizeof(double)) {
+        if (axis == 0)
+          hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
+        if constexpr (NDIM > 1)
+          if (axis == 1)
+            hipLaunchKernelGGL((k_axis<NDIM, NDIR, 1, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
+        if constexpr (NDIM > 2)
+          if (axis == 2)
+            hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
+      }
+      return;
+    }
     if (axis == 0) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0, S, C>), dim3(nb), dim3(bs), 0, st, A);
     if constexpr (NDIM > 1)
       if (axis == 1)

@@ -23,11 +23,11 @@ def test_header_declares_entry_points():
 
 
 def test_library_exports_every_declared_symbol():
-    from sac_amd import batch, device
+    from sac_amd import batch, closure, device
     L = device.load_library()
     missing = [s for s in declared_symbols() if not hasattr(L, s)]
     assert not missing, missing
-    assert set(device.EXPORTS) | set(batch.EXPORTS) == set(declared_symbols())
+    assert set(device.EXPORTS) | set(batch.EXPORTS) | set(closure.EXPORTS) == set(declared_symbols())
 
 
 def test_init_fails_loudly_without_gpu():
