@@ -11,7 +11,7 @@ resident in HBM when the timed region starts.
 
 Metric: Mcell-updates/s per RK stage = N_mesh * steps * 4 / wall time.
 Roofline: the unit is one RK stage, which the library issues as a fixed
-kernel sequence (k_axis for each axis, k_final, k_ghost); algorithmic bytes =
+kernel sequence (k_line for each axis, k_final, k_ghost); algorithmic bytes =
 296 B per mesh cell per stage (4*n_evolve*8 + n_background*8 for 8 evolving
 and 5 background FP64 fields) x N_mesh, divided by the summed average
 durations of that sequence, measured with HIP events on the library's
@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the tube-ensemble leg (4096 uneven 1-D tubes to t=0.1)")
     ap.add_argument("--tubes", type=int, default=4096)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-closure form (train + run at amplitude 0.1)")
+    ap.add_argument("--learned-steps", type=int, default=12)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run the library's plain copy kernel on one register "
                          "(known bytes) for HBM-counter calibration")
@@ -160,7 +163,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "RK stage = 3 x k_axis<3,3,K> + k_final<3,3> + k_ghost",
+                     "kernel": "RK stage = 3 x k_line<3,3,K> + k_final<3,3> + k_ghost",
                      "avg_ms": round(per_stage_ms, 4),
                      "launches": f_n},
         "stage_breakdown_ms": {"k_axis": round(a_ms / max(a_n, 1), 4),
@@ -193,6 +196,8 @@ def main():
                          "bytes_per_cell_stage": BYTES_PER_CELL_STAGE["mixed"]}}}
     if not args.no_batch:
         out.setdefault("forms", {})["batched_tubes"] = batched_leg(args.tubes)
+    if not args.no_learned:
+        out.setdefault("forms", {})["learned_closure"] = learned_leg(n, args.learned_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
@@ -234,6 +239,81 @@ def batched_leg(count):
             "failed": int((st == 2).sum()),
             "wall_roofline_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4),
             "bitwise_equal_to_single_tube_runs": checks}
+
+
+def learned_leg(n, steps):
+    """The learned hyperdiffusion closure on the stratified setup at
+    amplitude 0.1: a plain FP64 run of `steps` RK4 steps writes (stencil,
+    exact ratio) samples of every face of two of its states (~1e8 at 256^3),
+    the network is trained on the GPU (4000 Adam steps of 16384 samples),
+    then the same `steps` are run with the closure on (shock-sensor threshold
+    0.1), with the plain run's dt sequence, and both are timed.  Reported:
+    per-variable rel-L2 of the learned run against the plain FP64 run and
+    the speed ratio of the two runs."""
+    import numpy as np
+    from sac_amd import closure, device, problems
+    from tests.helpers import per_var_errors
+    prob = problems.stratified(n=n, amp=0.1)
+    ncell = n ** 3
+
+    def fresh():
+        d = device.Device(prob)
+        d.upload(prob.w)
+        d.getboundary()
+        return d
+
+    dev = fresh()
+    cl = closure.Closure(dev, capacity=2 * prob.ndim * ncell, max_blocks=256, seed=1)
+    dts = []
+    for i in range(steps):
+        dts.append(dev.step(0.3))
+        if i in (steps // 3, 2 * steps // 3):
+            cl.collect()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    losses = [cl.train(1, lr=1e-3)]
+    for k in range(8):
+        losses.append(cl.train(500, lr=1e-3 if k < 5 else 3e-4, base=(k + 1) * 500 * 256 * 64))
+    train_s = time.perf_counter() - t0
+    samples = cl.n
+    params = cl.params
+    dev.close()
+
+    def timed(learned):
+        d = fresh()
+        if learned:
+            c = closure.Closure(d, capacity=0, max_blocks=8, seed=1)
+            c.params = params
+            c.enable(True, 0.1)
+        d.advance(dts[0])  # warm-up of the launch path (not timed)
+        d.close()
+        d = fresh()
+        if learned:
+            c = closure.Closure(d, capacity=0, max_blocks=8, seed=1)
+            c.params = params
+            c.enable(True, 0.1)
+        d.synchronize()
+        t0 = time.perf_counter()
+        for dt in dts:
+            d.advance(dt)
+        d.synchronize()
+        el = time.perf_counter() - t0
+        w = d.download()
+        d.close()
+        return el, w
+
+    el_exact, w_exact = timed(False)
+    el_learned, w_learned = timed(True)
+    errs = per_var_errors(prob, w_learned, w_exact, 8)
+    return {"value": round(ncell * steps * 4 / el_learned / 1e6, 3), "unit": "Mcell-updates/s",
+            "workload": f"stratified {n}^3, perturbation amplitude 0.1 c_s, {steps} RK4 steps with "
+                        "the plain run's dt sequence, sensor threshold 0.1",
+            "speedup_vs_fp64": round(el_exact / el_learned, 3),
+            "rel_l2_vs_fp64_gpu": [float("%.3e" % e) for e in errs],
+            "finite": bool(np.isfinite(w_learned).all()),
+            "training": {"samples": int(samples), "adam_steps": 4000, "batch": 16384,
+                         "seconds": round(train_s, 3),
+                         "mse_first_last": [float("%.3e" % losses[0]), float("%.3e" % losses[-1])]}}
 
 
 def measured_traffic(precision, n):

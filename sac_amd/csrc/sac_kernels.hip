@@ -24,12 +24,30 @@
 //   gradient1/1L/1R    src/vacusr.viscosity.t:547-675
 //   getboundary        src/vacgrid.t:195-512
 #include "sac_device.h"
+#include "sac_line.h"
+
+// SAC_LINE_TILES=0 keeps the line-group kernel k_axis for every grid (the
+// default runs 2-D and 3-D grids through the LDS line tiles, k_line)
+#ifndef SAC_LINE_TILES
+#define SAC_LINE_TILES 1
+#endif
+
+// SAC_AXIS_WAVES: if set, the minimum number of waves per SIMD k_axis is
+// compiled for (the register budget follows: 512/waves VGPRs)
+#ifndef SAC_AXIS_WAVES
+#define SAC_AXIS_WAVES 0
+#endif
+#if SAC_AXIS_WAVES > 0
+#define SAC_AXIS_OCC __attribute__((amdgpu_waves_per_eu(SAC_AXIS_WAVES)))
+#else
+#define SAC_AXIS_OCC
+#endif
 
 namespace sac {
 
 // kernel 1: line groups of axis K (see axis_group)
 template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
-__global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
+__global__ void __launch_bounds__(256) SAC_AXIS_OCC k_axis(AxisArgs a) {
   constexpr int CK = LineGroup<K>::CK;
   const GridDesc& g = a.g;
   const long long t = xcd_block(a.xcd) * blockDim.x + threadIdx.x;
@@ -61,6 +79,48 @@ __global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
     double r2[1] = {(double)numax};
     block_max_atomic<1>(r2, a.red_visc);
   }
+}
+
+// SAC_LINE_WAVES: if set, the minimum waves per SIMD k_line is compiled for
+#ifndef SAC_LINE_WAVES
+#define SAC_LINE_WAVES 0
+#endif
+#if SAC_LINE_WAVES > 0
+#define SAC_LINE_OCC __attribute__((amdgpu_waves_per_eu(SAC_LINE_WAVES)))
+#else
+#define SAC_LINE_OCC
+#endif
+
+// kernel 1, 2-D and 3-D grids: LDS-staged line tiles of axis K (see line_tile)
+template <int NDIM, int NDIR, int K, typename S, typename C>
+__global__ void __launch_bounds__(line::NT) SAC_LINE_OCC k_line(AxisArgs a, int seg) {
+  __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::RR * line::W];
+  C numax = 0;
+  line::line_tile<NDIM, NDIR, K, S, C>(a, seg, L, numax);
+  if (a.visc_on) {
+    double r2[1] = {(double)numax};
+    block_max_atomic<1>(r2, a.red_visc);
+  }
+}
+
+// segment length along K: whole lines unless that leaves fewer than 2048
+// blocks (then halves, down to 32 cells)
+template <int NDIM, int K>
+static int line_segment(const GridDesc& g) {
+  const int ext = g.Mhi[K] - g.Mlo[K] + 1;
+  const long long lines = line::nblocks<NDIM, K>(g, ext);
+  int nseg = 1;
+  while (lines * nseg < 2048 && ext / (2 * nseg) >= 32) nseg *= 2;
+  return (ext + nseg - 1) / nseg;
+}
+
+template <int NDIM, int NDIR, int K, typename S, typename C>
+static void launch_line(const AxisArgs& A, hipStream_t st) {
+  const int seg = line_segment<NDIM, K>(A.g);
+  const long long nb = line::nblocks<NDIM, K>(A.g, seg);
+  if (nb > 0)
+    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C>), dim3((unsigned)nb), dim3(line::NT), 0, st, A,
+                       seg);
 }
 
 // kernel 2: the combining update of every mesh cell (see final_cell)
@@ -218,6 +278,13 @@ static void launch_nd(int which, int axis, const void* args, hipStream_t st, lon
           if (axis == 2)
             hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
       }
+      return;
+    }
+    if constexpr (NDIM > 1 && SAC_LINE_TILES) {
+      if (axis == 0) launch_line<NDIM, NDIR, 0, S, C>(A, st);
+      if (axis == 1) launch_line<NDIM, NDIR, 1, S, C>(A, st);
+      if constexpr (NDIM > 2)
+        if (axis == 2) launch_line<NDIM, NDIR, 2, S, C>(A, st);
       return;
     }
     if (axis == 0) hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0, S, C>), dim3(nb), dim3(bs), 0, st, A);

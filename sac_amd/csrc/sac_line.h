@@ -1,0 +1,319 @@
+// sac_line.h -- the per-axis work of an RK stage on 2-D and 3-D grids as
+// LDS-staged line tiles.
+//
+// The work is that of axis_group (sac_device.h): for the cells of lines
+// along axis K, the 4th-order flux divergence along K with the background
+// energy terms (centdiff4, src/vaccd.t:35-70), setnu for the 2*ndim+1
+// variables of axis K (src/vacusr.viscosity.t:224-455), the hyperdiffusion
+// line terms and the cross-term factors T (src/vacusr.viscosity.t:2-221).
+// The arithmetic of each value is the same; the schedule differs.
+//
+// Schedule.  A block of 256 threads owns W = 16 neighbouring lines and
+// marches along K in chunks of CH = 16 points.  Every thread loads ONE new
+// point per chunk (13 fields, coalesced along x), evaluates everything that
+// depends on that point alone (the flux and velocities along K, p_Tb and
+// the background field, rho_t, momenta and the seven setnu q values) once,
+// and stores it in a ring of CH+6 rows in LDS.  After a barrier the thread
+// assembles the cell three points behind its point from the ring: the
+// flux differences of points c-2..c+2 and the face ratios of the 7-point q
+// window c-3..c+3.  Each point is loaded from HBM and evaluated once per
+// axis, instead of once per neighbour that uses it.
+//
+// Ring layout: [value][row][line] for K > 0 (a wave reads 4 rows x 16
+// consecutive lines), [value][line][row] for K = 0 (a wave reads 16
+// consecutive rows of 4 lines), both free of LDS bank conflicts.
+//
+// Grid edges along K follow axis_group: the q layer beyond the grid is the
+// mirror of the cell 6 inward (src/vacusr.viscosity.t:378-400); points past
+// it are loaded from that same in-grid cell and never reach a mesh cell's
+// result.
+#pragma once
+#include "sac_device.h"
+
+namespace sac {
+namespace line {
+
+#ifndef SAC_LINE_CH
+#define SAC_LINE_CH 16
+#endif
+constexpr int CH = SAC_LINE_CH;      // points per chunk along K
+constexpr int W = 16;                // lines per block
+constexpr int HALO = 3;              // reach of the setnu stencil
+constexpr int RR = CH + 2 * HALO;    // ring rows
+constexpr int NT = CH * W;           // threads per block
+
+template <int NDIM, int NDIR>
+struct Slots {
+  static constexpr int NE = 2 + 2 * NDIR;
+  static constexpr int NP = 2 * NDIM + 1;
+  static constexpr int F0 = 0;          // flux along K of the NE evolving fields
+  static constexpr int V0 = F0 + NE;    // v_l, l < NDIM
+  static constexpr int PB = V0 + NDIM;  // p_Tb (getptotal_bg)
+  static constexpr int B0 = PB + 1;     // bg_l, l < NDIM
+  static constexpr int RT = B0 + NDIM;  // rho_t
+  static constexpr int M0 = RT + 1;     // m_l, l < NDIM
+  static constexpr int Q0 = M0 + NDIM;  // setnu q of the NP variables
+  static constexpr int NV = Q0 + NP;
+};
+
+template <int K>
+__device__ __forceinline__ int at(int v, int row, int col) {
+  return (K == 0) ? (v * W + col) * RR + row : (v * RR + row) * W + col;
+}
+
+// Axes of a block: A carries the W lines of the block, B (3-D only) one
+// line set per block index.
+template <int NDIM, int K>
+struct Axes {
+  static constexpr int A = (K == 0) ? 1 : 0;
+  static constexpr int B = (NDIM == 3) ? ((K == 2) ? 1 : 2) : -1;
+};
+
+// blocks of a launch for segments of `seg` cells along K
+template <int NDIM, int K>
+__host__ __device__ inline long long nblocks(const GridDesc& g, int seg) {
+  constexpr int A = Axes<NDIM, K>::A, B = Axes<NDIM, K>::B;
+  const long long nA = (g.Mhi[A] - g.Mlo[A] + 1 + W - 1) / W;
+  const long long nB = (B >= 0) ? (g.Mhi[B] - g.Mlo[B] + 1) : 1;
+  const long long nK = (g.Mhi[K] - g.Mlo[K] + 1 + seg - 1) / seg;
+  return nA * nB * nK;
+}
+
+template <int NDIM, int NDIR, int K, typename S, typename C>
+__device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restrict__ L,
+                                          C& numax) {
+  using SL = Slots<NDIM, NDIR>;
+  constexpr int NE = SL::NE, NP = SL::NP;
+  constexpr int A = Axes<NDIM, K>::A, B = Axes<NDIM, K>::B;
+  const GridDesc& g = a.g;
+  const long long fs = g.fs, sk = g.st[K];
+  const S* __restrict__ Wf = (const S*)a.wct;
+  const double* __restrict__ Bg = a.bg;
+  const double gamma = a.gamma;
+  const int GK = g.G[K], MloK = g.Mlo[K], MhiK = g.Mhi[K];
+
+  const int tid = threadIdx.x;
+  const int ik = (K == 0) ? (tid % CH) : (tid / W);  // position in the chunk
+  const int iw = (K == 0) ? (tid / CH) : (tid % W);  // line of the block
+
+  // block -> (line tile along A, line along B, segment along K)
+  long long b = blockIdx.x;
+  const int nA = (g.Mhi[A] - g.Mlo[A] + 1 + W - 1) / W;
+  const int bA = (int)(b % nA);
+  b /= nA;
+  int cB = 0;
+  if constexpr (B >= 0) {
+    const int nB = g.Mhi[B] - g.Mlo[B] + 1;
+    cB = g.Mlo[B] + (int)(b % nB);
+    b /= nB;
+  }
+  const int sIdx = (int)b;
+  int cA = g.Mlo[A] + bA * W + iw;
+  const bool line_ok = cA <= g.Mhi[A];
+  if (!line_ok) cA = g.Mhi[A];  // computed on an in-grid line, never stored
+  int c3[3] = {0, 0, 0};
+  c3[A] = cA;
+  if constexpr (B >= 0) c3[B] = cB;
+  const long long base = lin(g, c3[0], c3[1], c3[2]);  // line origin (index 0 along K)
+
+  const int cs = MloK + sIdx * seg;                      // first cell of the segment
+  const int ce = (cs + seg - 1 < MhiK) ? cs + seg - 1 : MhiK;
+  const int ps = cs - HALO;                              // first point of the ring
+  const int nch = (ce - cs + 1 + 2 * HALO + CH - 1) / CH;
+
+  const C ctk = a.visc_on ? (C)red_get(a.ctot_src, K) : (C)0;
+  S* __restrict__ Racc = (S*)a.R;
+  S* __restrict__ Tout = (S*)a.T;
+
+  // global index of the point of chunk j (mirror layer beyond the grid)
+  auto point_index = [&](int j) {
+    int pK = ps + j * CH + ik;
+    if (pK < 0) pK = MloK + 3;       // mirror layer below the grid
+    if (pK > GK - 1) pK = MhiK - 3;  // and above it
+    return base + (long long)pK * sk;
+  };
+  // global index of the cell of chunk j (clamped into the grid; only cells
+  // of the segment are stored)
+  auto cell_clamped = [&](int j) {
+    const int c = ps + j * CH + ik - HALO;
+    return (c < 0) ? 0 : ((c > GK - 1) ? GK - 1 : c);
+  };
+  auto cell_index = [&](int j) { return base + (long long)cell_clamped(j) * sk; };
+  // Software pipeline: the point, the cell width and (K > 0) the rate
+  // accumulator of chunk j+1 are loaded while chunk j is evaluated.  The
+  // cell's loads are issued first so that waiting for them never waits for
+  // the point loads behind them.
+  Pt<NDIR, C> nxt;
+  C Rn[NE];
+  double dxn = a.dx[K][cell_clamped(0)];
+  if (K > 0) {
+    const long long ic = cell_index(0);
+#pragma unroll
+    for (int n = 0; n < NE; n++) Rn[n] = (C)Racc[n * fs + ic];
+  }
+  load_pt<NDIR, S, C>(Wf, Bg, fs, point_index(0), nxt);
+
+  int rb = 0;  // ring row of the chunk's first point
+  for (int j = 0; j < nch; j++) {
+    const int o = j * CH + ik;  // point offset from ps
+    int r0 = rb + ik;
+    if (r0 >= RR) r0 -= RR;
+    const Pt<NDIR, C> pt = nxt;
+    C Rold[NE];
+#pragma unroll
+    for (int n = 0; n < NE; n++) Rold[n] = (K > 0) ? Rn[n] : (C)0;
+    const double dxd = dxn;
+    if (j + 1 < nch) {
+      dxn = a.dx[K][cell_clamped(j + 1)];
+      if (K > 0) {
+        const long long ic = cell_index(j + 1);
+#pragma unroll
+        for (int n = 0; n < NE; n++) Rn[n] = (C)Racc[n * fs + ic];
+      }
+      load_pt<NDIR, S, C>(Wf, Bg, fs, point_index(j + 1), nxt);
+    }
+    // ---- the new point: evaluate what depends on it alone ----
+    {
+      C F[NE], v[NDIM];
+      flux_pt<NDIR, NDIM, C>(pt, K, gamma, F, v);
+#pragma unroll
+      for (int n = 0; n < NE; n++) L[at<K>(SL::F0 + n, r0, iw)] = F[n];
+#pragma unroll
+      for (int l = 0; l < NDIM; l++) {
+        L[at<K>(SL::V0 + l, r0, iw)] = v[l];
+        L[at<K>(SL::B0 + l, r0, iw)] = (C)pt.bg[l];
+        L[at<K>(SL::M0 + l, r0, iw)] = pt.m[l];
+      }
+      L[at<K>(SL::PB, r0, iw)] = (C)ptotal_bg<NDIR, C>(pt, gamma);
+      const double rtd = (double)pt.rho + pt.rhob;
+      L[at<K>(SL::RT, r0, iw)] = (C)rtd;
+      {
+#pragma clang fp contract(off)
+        // q of setnu: rho', thermal energy (in FP64), v_l, b_l (l != K)
+        const double inv = recip<C>(rtd);
+        double sb = 0.0, sm = 0.0;
+#pragma unroll
+        for (int cd = 0; cd < NDIR; cd++) {
+          const double mc = (double)pt.m[cd], bc = (double)pt.b[cd];
+          sb += bc * bc;
+          sm += mc * mc;
+        }
+        L[at<K>(SL::Q0 + 0, r0, iw)] = pt.rho;
+        L[at<K>(SL::Q0 + 1, r0, iw)] = (C)((double)pt.e - 0.5 * (sb + sm * inv));
+#pragma unroll
+        for (int l = 0; l < NDIM; l++) L[at<K>(SL::Q0 + 2 + l, r0, iw)] = pt.m[l] * (C)inv;
+#pragma unroll
+        for (int n = 0; n < NDIM - 1; n++) {
+          const int lb = (n < K) ? n : n + 1;
+          L[at<K>(SL::Q0 + 2 + NDIM + n, r0, iw)] = pt.b[lb];
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- the cell three points behind: c = ps + o - 3 ----
+    {
+      const int c = ps + o - HALO;
+      const bool valid = line_ok && c >= cs && c <= ce;
+      auto row = [&](int d) {  // ring row of point c + d
+        int r = r0 - HALO + d;
+        return (r < 0) ? r + RR : r;
+      };
+      const C dxk = (C)dxd, idx = (C)(1.0 / dxd);
+      const long long icell = cell_index(j);
+      C R[NE];
+      {
+        const bool fluxc = c >= MloK + 2 && c <= MhiK - 2;
+        const int rm2 = row(-2), rm1 = row(-1), rp1 = row(1), rp2 = row(2), r00 = row(0);
+        const C inv12 = idx / (C)12;
+#pragma unroll
+        for (int n = 0; n < NE; n++) {
+          C E1, e2;
+          const C D1 = two_diff(L[at<K>(SL::F0 + n, rp1, iw)], L[at<K>(SL::F0 + n, rm1, iw)], E1);
+          const C D2 = two_diff(L[at<K>(SL::F0 + n, rp2, iw)], L[at<K>(SL::F0 + n, rm2, iw)], e2);
+          R[n] = ((D2 - (C)8 * D1) + (e2 - (C)8 * E1)) * inv12;
+        }
+        const C pTb0 = L[at<K>(SL::PB, r00, iw)];
+        const C bgk = L[at<K>(SL::B0 + K, r00, iw)];
+        C re = 0;
+#pragma unroll
+        for (int l = 0; l < NDIM; l++) {
+          const C dv1 = L[at<K>(SL::V0 + l, rp1, iw)] - L[at<K>(SL::V0 + l, rm1, iw)];
+          const C dv2 = L[at<K>(SL::V0 + l, rp2, iw)] - L[at<K>(SL::V0 + l, rm2, iw)];
+          const C Dv = -(dv2 - (C)8 * dv1) * inv12;
+          if (l == K) re -= Dv * pTb0;
+          re += bgk * L[at<K>(SL::B0 + l, r00, iw)] * Dv;
+        }
+        R[1 + NDIR] += re;
+#pragma unroll
+        for (int n = 0; n < NE; n++) R[n] = fluxc ? R[n] : (C)0;
+      }
+      if (a.visc_on) {
+#pragma clang fp contract(off)
+        const bool z1 = (c == MloK && !g.per_lo[K]) || (c == MhiK && !g.per_hi[K]);
+        const int rw[7] = {row(-3), row(-2), row(-1), row(0), row(1), row(2), row(3)};
+        const C rtm = L[at<K>(SL::RT, rw[2], iw)], rt0 = L[at<K>(SL::RT, rw[3], iw)],
+                rtp = L[at<K>(SL::RT, rw[4], iw)];
+        const C rhoL = (rt0 + rtm) * (C)0.5, rhoR = (rtp + rt0) * (C)0.5;
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+          C q[7];
+#pragma unroll
+          for (int s = 0; s < 7; s++) q[s] = L[at<K>(SL::Q0 + p, rw[s], iw)];
+          const C ch = (p >= 2 && p < 2 + NDIM) ? (C)0.4 : (C)0.04;
+          const C Kc = ctk * ch;
+          const C nuL = Kc * face_ratio<C>(q) * dxk;
+          const C nuR = Kc * face_ratio<C>(q + 1) * dxk;
+          numax = valid ? fmax(numax, fmax(nuR, nuL)) : numax;
+          const C qm = q[2], q0 = q[3], qp = q[4];
+          const C gL = z1 ? (C)0 : (q0 - qm) * idx;
+          const C gR = z1 ? (C)0 : (qp - q0) * idx;
+          const C tL = nuL * gL, tR = nuR * gR;
+          if (p == 0) {
+            R[0] += (tR - tL) * idx;
+          } else if (p == 1) {
+            R[1 + NDIR] += (tR - tL) * idx;
+          } else if (p < 2 + NDIM) {
+            const int l = p - 2;
+            const C m0 = L[at<K>(SL::M0 + l, rw[3], iw)];
+            const C mL = (m0 + L[at<K>(SL::M0 + l, rw[2], iw)]) * (C)0.5;
+            const C mR = (L[at<K>(SL::M0 + l, rw[4], iw)] + m0) * (C)0.5;
+            const C mult = (l == K) ? (C)1 : (C)0.5;  // the i==k branch runs twice for l==k
+            R[1 + l] += mult * (rhoR * tR - rhoL * tL) * idx;
+            R[1 + NDIR] += mult * (mR * tR - mL * tL) * idx;
+            if (l != K && valid) {
+              const C gC = z1 ? (C)0 : (qp - qm) * idx * (C)0.5;
+              Tout[(long long)cross_index(NDIM, K, l) * fs + icell] = (S)(gC * (nuL + nuR) * (C)0.25);
+            }
+          } else {
+            const int n = p - 2 - NDIM;
+            const int lb = (n < K) ? n : n + 1;
+            const C BL = (q0 + qm) * (C)0.5, BR = (qp + q0) * (C)0.5;
+            R[2 + NDIR + lb] += (tR - tL) * idx;
+            R[1 + NDIR] += (tR * BR - tL * BL) * idx;
+            if (valid) {
+              const C gC = z1 ? (C)0 : (qp - qm) * idx * (C)0.5;
+              Tout[(long long)(cross_index(NDIM, K, lb) + 1) * fs + icell] =
+                  (S)(gC * (nuL + nuR) * (C)0.5);
+            }
+          }
+        }
+      }
+      if (valid) {
+        if (K == 0) {
+#pragma unroll
+          for (int n = 0; n < NE; n++) Racc[n * fs + icell] = (S)R[n];
+        } else {
+#pragma unroll
+          for (int n = 0; n < NE; n++) Racc[n * fs + icell] = (S)(Rold[n] + R[n]);
+        }
+      }
+    }
+    __syncthreads();  // the next chunk overwrites ring rows read above
+    rb += CH;
+    if (rb >= RR) rb -= RR;
+  }
+}
+
+}  // namespace line
+}  // namespace sac
