@@ -446,8 +446,20 @@ program sac_vac
   end do
   nxc = 1
   nxc(1:ndim) = nx(1:ndim)
-  adv = SAC_ADV_FOURSTEP
-  if (typeadvance == 'onestep') adv = SAC_ADV_ONESTEP
+  select case (typeadvance)
+  case ('onestep')
+    adv = SAC_ADV_ONESTEP
+  case ('twostep')
+    adv = SAC_ADV_TWOSTEP
+  case ('threestep')
+    adv = SAC_ADV_THREESTEP
+  case ('sterck')
+    adv = SAC_ADV_STERCK
+  case ('jameson')
+    adv = SAC_ADV_JAMESON
+  case default
+    adv = SAC_ADV_FOURSTEP
+  end select
   rc = sac_dev_init(int(ndim, c_int), int(ndir, c_int), nxc, int(nw, c_int), iws, &
        int(nevolve, c_int), eqpar, int(neqpar + nspecialpar, c_int), int(grav0_, c_int), &
        int(nu_, c_int), dxaxis, bcodes, merge(1_c_int, 0_c_int, fullgridini), adv, &
@@ -530,6 +542,14 @@ contains
       bc_code = SAC_BC_SYMM0
     case ('asymm')
       bc_code = SAC_BC_ASYMM
+    case ('cont1')
+      bc_code = SAC_BC_CONT1
+    case ('fixed1')
+      bc_code = SAC_BC_FIXED1
+    case ('grad1')
+      bc_code = SAC_BC_GRAD1
+    case ('contCD4')
+      bc_code = SAC_BC_CONTCD4
     case default
       call die('Error in GetBoundary: boundary type ' // trim(s) // ' is not implemented!')
       bc_code = 0
@@ -631,8 +651,13 @@ contains
     sourcesplit = typesourcesplit /= 'unsplit'
     if (sourcesplit) sourceunsplit = .false.
     if (typeadvance == 'twostep' .and. typepred1 == 'nul') typeadvance = 'onestep'
-    if (typeadvance /= 'fourstep' .and. typeadvance /= 'onestep') &
-         call die('typeadvance=' // trim(typeadvance) // ' is not built')
+    if (typeadvance == 'twostep' .and. typepred1 /= 'cd4') &
+         call die('typepred=' // trim(typepred1) // ' is not built (twostep needs cd4 or nul)')
+    select case (typeadvance)
+    case ('onestep', 'twostep', 'threestep', 'fourstep', 'sterck', 'jameson')
+    case default
+      call die('typeadvance=' // trim(typeadvance) // ' is not built')
+    end select
     if (dimsplit) call die('dimsplit=T is not built: set dimsplit=F')
     if (sourcesplit .or. .not. sourceunsplit) call die('only sourceunsplit=T is built')
     if (divbfix .and. ndim > 1) call die('divbfix=T (Powell source) is not built: set divbfix=F')

@@ -37,10 +37,18 @@ extern "C" {
 #define SAC_BC_SYMM 4
 #define SAC_BC_SYMM0 5
 #define SAC_BC_ASYMM 6
+#define SAC_BC_CONT1 7   /* first-order extrapolation (src/vacgrid.t:376-407) */
+#define SAC_BC_FIXED1 8  /* extrapolated once, then restored */
+#define SAC_BC_GRAD1 9   /* extrapolated once, then stored difference + edge value */
+#define SAC_BC_CONTCD4 10 /* mirror about the edge cell (src/vacgrid.t:260-290) */
 
 /* typeadvance codes (src/vac.t:406-480) */
 #define SAC_ADV_ONESTEP 1
+#define SAC_ADV_TWOSTEP 2   /* with a 'cd4' predictor; typepred1='nul' is onestep */
+#define SAC_ADV_THREESTEP 3
 #define SAC_ADV_FOURSTEP 4
+#define SAC_ADV_STERCK 5
+#define SAC_ADV_JAMESON 6
 
 /* precision of the evolving fields */
 #define SAC_PREC_FP64 0  /* everything in FP64 */
@@ -84,7 +92,8 @@ typedef void *sac_handle;
  *   typeB[iw*2*ndim+iB]  boundary code per variable and boundary region
  *                   (regions ordered lower/upper per axis, src/vacgrid.t:53-69)
  *   fullgridini     1 if the ini file carried ghost cells
- *   typeadvance     SAC_ADV_FOURSTEP or SAC_ADV_ONESTEP
+ *   typeadvance     SAC_ADV_* (onestep, twostep, threestep, fourstep, sterck,
+ *                   jameson of src/vac.t:406-475)
  *   precision       SAC_PREC_FP64 or SAC_PREC_MIXED (the host arrays stay FP64
  *                   either way; FP32 storage is converted at upload/download)
  */

@@ -18,7 +18,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SAC_ORACLE_LIB") or os.path.join(_HERE, "_build", "libsac_oracle.so")
 
-BC = {"cont": 1, "fixed": 2, "periodic": 3, "symm": 4, "symm0": 5, "asymm": 6}
+BC = {"cont": 1, "fixed": 2, "periodic": 3, "symm": 4, "symm0": 5, "asymm": 6,
+      "cont1": 7, "fixed1": 8, "grad1": 9, "contCD4": 10}
 ADV = {"onestep": 1, "twostep": 2, "threestep": 3, "fourstep": 4,
        "sterck": 5, "jameson": 6}
 

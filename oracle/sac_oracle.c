@@ -408,7 +408,8 @@ int oracle_get_dx(const sac_oracle *o, double *dx) {
 #define DX(idim) (o->dx[(long)((idim)-1) * o->n + IDX(o, i1, i2, i3)])
 
 /* ------------------------------------------------------------------ */
-/* getboundary (src/vacgrid.t:195-512) for cont/fixed/periodic/symm/asymm */
+/* getboundary (src/vacgrid.t:195-512) for cont/fixed/periodic/symm/asymm,
+ * cont1/fixed1/grad1 and contCD4 */
 
 static void getboundary(sac_oracle *o, double *w) {
   int nd = o->ndim;
@@ -441,6 +442,40 @@ static void getboundary(sac_oracle *o, double *w) {
             F(w, iw) = FSH(w, iw, idim, ixe - ix);
           }
         }
+      } else if (tb == OR_BC_CONT1 || tb == OR_BC_FIXED1 || tb == OR_BC_GRAD1) {
+        /* first-order extrapolation from the edge and the inner edge
+         * (src/vacgrid.t:376-407); fixed1 restores stored values, grad1
+         * adds stored differences to the current edge value */
+        int ixe = upper ? r.lo[idim - 1] - 1 : r.hi[idim - 1] + 1;
+        int ixf = upper ? ixe - 1 : ixe + 1;
+        if (o->fixedB[iw - 1][iB]) {
+          const double *fb = o->fixB + ((long)iB * o->nw + (iw - 1)) * o->n;
+          if (tb == OR_BC_GRAD1) {
+            FOR_BOX(r) {
+              int ix = (idim == 1) ? i1 : (idim == 2) ? i2 : i3;
+              F(w, iw) = C(fb) + FSH(w, iw, idim, ixe - ix);
+            }
+          } else {
+            FOR_BOX(r) { F(w, iw) = C(fb); }
+          }
+        } else if (tb == OR_BC_CONT1 || !o->fullgridini) {
+          FOR_BOX(r) {
+            int ix = (idim == 1) ? i1 : (idim == 2) ? i2 : i3;
+            int k = abs(ix - ixe);
+            F(w, iw) = (k + 1) * FSH(w, iw, idim, ixe - ix) - k * FSH(w, iw, idim, ixf - ix);
+          }
+        }
+      } else if (tb == OR_BC_CONTCD4) {
+        /* mirror about the edge cell for two ghost layers (src/vacgrid.t:260-290):
+         * upper ghosts ixmin <- ixmin-2, ixmax <- ixmin-3; lower ghosts
+         * ixmin <- ixmax+3, ixmax <- ixmax+2 */
+        int lo = r.lo[idim - 1], hi = r.hi[idim - 1];
+        int src_lo = upper ? lo - 2 : hi + 3, src_hi = upper ? lo - 3 : hi + 2;
+        FOR_BOX(r) {
+          int ix = (idim == 1) ? i1 : (idim == 2) ? i2 : i3;
+          if (ix == lo) F(w, iw) = FSH(w, iw, idim, src_lo - ix);
+          else if (ix == hi) F(w, iw) = FSH(w, iw, idim, src_hi - ix);
+        }
       } else if (tb == OR_BC_SYMM || tb == OR_BC_SYMM0 || tb == OR_BC_ASYMM) {
         int ixe = upper ? 2 * r.lo[idim - 1] - 1 : 2 * r.hi[idim - 1] + 1;
         FOR_BOX(r) {
@@ -465,10 +500,21 @@ static void getboundary(sac_oracle *o, double *w) {
         r.hi[idim - 1] = o->Mlo[idim - 1] - 1;
       }
       for (int iw = 1; iw <= o->nw; iw++) {
-        if (o->typeB[iw - 1][iB] == OR_BC_FIXED && !o->fixedB[iw - 1][iB]) {
+        int tb = o->typeB[iw - 1][iB];
+        if ((tb == OR_BC_FIXED || tb == OR_BC_FIXED1 || tb == OR_BC_GRAD1) &&
+            !o->fixedB[iw - 1][iB]) {
           o->fixedB[iw - 1][iB] = 1;
           double *fb = o->fixB + ((long)iB * o->nw + (iw - 1)) * o->n;
-          FOR_BOX(r) { C(fb) = F(w, iw); }
+          if (tb == OR_BC_GRAD1) {
+            /* differences to the edge value (src/vacgrid.t:489-494) */
+            int ixe = upper ? r.lo[idim - 1] - 1 : r.hi[idim - 1] + 1;
+            FOR_BOX(r) {
+              int ix = (idim == 1) ? i1 : (idim == 2) ? i2 : i3;
+              C(fb) = F(w, iw) - FSH(w, iw, idim, ixe - ix);
+            }
+          } else {
+            FOR_BOX(r) { C(fb) = F(w, iw); }
+          }
         }
       }
     }

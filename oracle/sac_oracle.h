@@ -31,7 +31,11 @@ enum {
   OR_BC_PERIODIC = 3,
   OR_BC_SYMM = 4,
   OR_BC_SYMM0 = 5,
-  OR_BC_ASYMM = 6
+  OR_BC_ASYMM = 6,
+  OR_BC_CONT1 = 7,
+  OR_BC_FIXED1 = 8,
+  OR_BC_GRAD1 = 9,
+  OR_BC_CONTCD4 = 10
 };
 
 /* typeadvance codes (src/vac.t:406-480) */

@@ -89,7 +89,6 @@ struct Dev {
   void* rate = nullptr;     // R accumulator
   void* cross = nullptr;    // cross-term factors T
   void* snap = nullptr;     // 'fixed' values of the evolving fields
-  double* bgsnap = nullptr;
   double* dx[3] = {nullptr, nullptr, nullptr};
   int* bc_e = nullptr;      // [ne][6]
   int* bc_b = nullptr;      // [nbg][6]
@@ -348,7 +347,7 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   if (neqpar < 2) return fail(SAC_E_ARG, "neqpar=%d < 2", neqpar);
   if (eqpar[0] <= 0.0) return fail(SAC_E_ARG, "gamma must be > 0");
   if (eqpar[1] != 0.0) return fail(SAC_E_UNSUPPORTED, "resistive MHD (eta /= 0) is not built");
-  if (typeadvance != SAC_ADV_FOURSTEP && typeadvance != SAC_ADV_ONESTEP)
+  if (typeadvance < SAC_ADV_ONESTEP || typeadvance > SAC_ADV_JAMESON)
     return fail(SAC_E_UNSUPPORTED, "typeadvance code %d not built", typeadvance);
   if (precision != SAC_PREC_FP64 && precision != SAC_PREC_MIXED)
     return fail(SAC_E_UNSUPPORTED, "precision %d not built", precision);
@@ -357,7 +356,7 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   for (int iw = 0; iw < nw; iw++)
     for (int iB = 0; iB < 2 * ndim; iB++) {
       int c = typeB[iw * 2 * ndim + iB];
-      if (c < SAC_BC_CONT || c > SAC_BC_ASYMM)
+      if (c < SAC_BC_CONT || c > SAC_BC_CONTCD4)
         return fail(SAC_E_UNSUPPORTED, "boundary code %d (var %d, region %d)", c, iw + 1, iB + 1);
     }
   // periodic pairing must be consistent (src/vacgrid.t:109-120)
@@ -424,7 +423,9 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   d->visc_on = std::fabs(nu) > kSmallDouble;
   for (int iw = 0; iw < nw; iw++)
     for (int iB = 0; iB < 2 * ndim; iB++)
-      if (typeB[iw * 2 * ndim + iB] == SAC_BC_FIXED) d->any_fixed = true;
+      if (typeB[iw * 2 * ndim + iB] == SAC_BC_FIXED || typeB[iw * 2 * ndim + iB] == SAC_BC_FIXED1 ||
+          typeB[iw * 2 * ndim + iB] == SAC_BC_GRAD1)
+        d->any_fixed = true;
 
   int rc = 0;
   auto bail = [&](int code) {
@@ -441,7 +442,6 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
     return bail(rc);
   if (d->any_fixed) {
     if ((rc = ealloc(*d, &d->snap, (long long)ne * g.fs))) return bail(rc);
-    if ((rc = dalloc(*d, &d->bgsnap, (long long)d->nbg * g.fs))) return bail(rc);
   }
   // dx per axis, dxmin per axis over the mesh
   long long off = 0;
@@ -507,7 +507,6 @@ int sac_dev_free(sac_handle h) {
   hipFree(d->rate);
   hipFree(d->cross);
   hipFree(d->snap);
-  hipFree(d->bgsnap);
   for (int k = 0; k < 3; k++) hipFree(d->dx[k]);
   hipFree(d->bc_e);
   hipFree(d->bc_b);
@@ -580,10 +579,15 @@ int sac_dev_getboundary(sac_handle h) {
     sac::launch_ghost(a, d.stream);
     ghost_fill_evolving(d, SAC_SLOT_W);
     if (d.any_fixed) {
-      HIPCHK(hipMemcpyAsync(d.snap, d.reg[SAC_SLOT_W], d.esz * (size_t)d.ne * d.g.fs,
-                            hipMemcpyDeviceToDevice, d.stream));
-      HIPCHK(hipMemcpyAsync(d.bgsnap, d.bg, sizeof(double) * (size_t)d.nbg * d.g.fs,
-                            hipMemcpyDeviceToDevice, d.stream));
+      // fixed/fixed1 values and grad1 differences of the first fill
+      // (src/vacgrid.t:473-507); the background is filled only once
+      sac::GhostArgs st{};
+      st.g = d.g;
+      st.prec = d.prec;
+      st.f = d.reg[SAC_SLOT_W];
+      st.bc = d.bc_e;
+      st.nvar = d.ne;
+      sac::launch_ghost_store(st, d.snap, d.stream);
     }
     d.initialized = true;
   } else {
@@ -639,21 +643,61 @@ int sac_dev_advance(sac_handle h, double dt) {
   if (!d.initialized) return fail(SAC_E_ARG, "getboundary must run before advance");
   if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SAC_E_ARG, "dt=%g", dt);
   const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, A = SAC_SLOT_ACC;
-  if (d.typeadvance == SAC_ADV_FOURSTEP) {
-    // classical RK4 (src/vac.t:439-455) in two registers plus accumulator
-    stage(d, W, W, S1, dt / 2, W, A, dt / 6);
-    stage(d, S1, W, S2, dt / 2, A, A, dt / 3);
-    stage(d, S2, W, S1, dt, A, A, dt / 3);
-    stage(d, S1, A, W, dt / 6, -1, -1, 0.0);
-  } else {
-    // onestep: w1=w; advect1(dt, wCT=w1, w) (src/vac.t:407-408)
-    HIPCHK(hipMemcpyAsync(d.reg[S1], d.reg[W], d.esz * (size_t)d.ne * d.g.fs,
-                          hipMemcpyDeviceToDevice, d.stream));
-    d.red_valid[S1] = d.red_valid[W];
-    if (d.red_valid[W])
-      HIPCHK(hipMemcpyAsync(red_of(d, S1), red_of(d, W), sizeof(unsigned long long) * sac::kRedSlots,
-                            hipMemcpyDeviceToDevice, d.stream));
-    stage(d, S1, W, W, dt, -1, -1, 0.0);
+  // The integrators of src/vac.t:406-475.  A stage is
+  // out = base + qa*R(wct) (acc_out = acc_in + qb*R), followed by the ghost
+  // fill of out, as advect1 ends with getboundary (src/vac.t:544).
+  const size_t nreg = (size_t)d.ne * d.g.fs;
+  switch (d.typeadvance) {
+    case SAC_ADV_FOURSTEP:
+      // classical RK4 (:439-455) in two registers plus accumulator
+      stage(d, W, W, S1, dt / 2, W, A, dt / 6);
+      stage(d, S1, W, S2, dt / 2, A, A, dt / 3);
+      stage(d, S2, W, S1, dt, A, A, dt / 3);
+      stage(d, S1, A, W, dt / 6, -1, -1, 0.0);
+      break;
+    case SAC_ADV_TWOSTEP:
+      // predictor advect1(dt/2, w -> w1), full step advect1(dt, wCT=w1, w) (:424-428)
+      stage(d, W, W, S1, dt / 2, -1, -1, 0.0);
+      stage(d, S1, W, W, dt, -1, -1, 0.0);
+      break;
+    case SAC_ADV_THREESTEP:
+      // Shu's third-order method (:430-436): w1 = w + dt R(w);
+      // w2 = 3/4 w + 1/4 w1 + dt/4 R(w1); w = w/3 + 2/3 w2 + 2dt/3 R(w2)
+      stage(d, W, W, S1, dt, -1, -1, 0.0);
+      sac::launch_axpy4(d.prec, (long long)nreg, d.reg[S2], d.reg[W], 0.75, d.reg[S1], 0.25,
+                        nullptr, 0.0, nullptr, 0.0, d.stream);
+      d.red_valid[S2] = false;
+      stage(d, S1, S2, S2, dt / 4, -1, -1, 0.0);
+      sac::launch_axpy4(d.prec, (long long)nreg, d.reg[W], d.reg[W], 1.0 / 3.0, d.reg[S2],
+                        2.0 / 3.0, nullptr, 0.0, nullptr, 0.0, d.stream);
+      d.red_valid[W] = false;
+      stage(d, S2, W, W, dt * 2.0 / 3.0, -1, -1, 0.0);
+      break;
+    case SAC_ADV_STERCK:
+      // (:458-466); 0.12 is a default-real (single precision) literal there
+      stage(d, W, W, S1, dt * (double)0.12f, -1, -1, 0.0);
+      stage(d, S1, W, S2, dt / 4, -1, -1, 0.0);
+      stage(d, S2, W, S1, dt / 2, -1, -1, 0.0);
+      stage(d, S1, W, W, dt, -1, -1, 0.0);
+      break;
+    case SAC_ADV_JAMESON:
+      // (:468-475)
+      stage(d, W, W, S1, dt / 4, -1, -1, 0.0);
+      stage(d, S1, W, S2, dt / 3, -1, -1, 0.0);
+      stage(d, S2, W, S1, dt / 2, -1, -1, 0.0);
+      stage(d, S1, W, W, dt, -1, -1, 0.0);
+      break;
+    default: {
+      // onestep: w1=w; advect1(dt, wCT=w1, w) (:407-408)
+      HIPCHK(hipMemcpyAsync(d.reg[S1], d.reg[W], d.esz * nreg, hipMemcpyDeviceToDevice,
+                            d.stream));
+      d.red_valid[S1] = d.red_valid[W];
+      if (d.red_valid[W])
+        HIPCHK(hipMemcpyAsync(red_of(d, S1), red_of(d, W),
+                              sizeof(unsigned long long) * sac::kRedSlots,
+                              hipMemcpyDeviceToDevice, d.stream));
+      stage(d, S1, W, W, dt, -1, -1, 0.0);
+    }
   }
   HIPCHK(hipGetLastError());
   return 0;

@@ -22,7 +22,18 @@ constexpr int kRedSlots = 8;                 // reduction values per buffer
 enum PrecCode : int { PREC_FP64 = 0, PREC_MIXED = 1 };
 
 // boundary codes (typeB strings, src/vacgrid.t:257-464)
-enum BcCode : int { BC_CONT = 1, BC_FIXED = 2, BC_PERIODIC = 3, BC_SYMM = 4, BC_SYMM0 = 5, BC_ASYMM = 6 };
+enum BcCode : int {
+  BC_CONT = 1,
+  BC_FIXED = 2,
+  BC_PERIODIC = 3,
+  BC_SYMM = 4,
+  BC_SYMM0 = 5,
+  BC_ASYMM = 6,
+  BC_CONT1 = 7,
+  BC_FIXED1 = 8,
+  BC_GRAD1 = 9,
+  BC_CONTCD4 = 10
+};
 
 struct GridDesc {
   int ndim, ndir;

@@ -649,36 +649,91 @@ __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g
 // the stored value of a 'fixed' region), which is what every thread does
 // here; no thread reads a ghost cell, so the kernel is race-free.
 
+// Value of ghost cell c of variable var after the reference's sequential
+// region fills (src/vacgrid.t:195-512): the region of the highest axis along
+// which c is a ghost writes last, and its sources lie in the mesh along that
+// axis, so they are evaluated the same way on the lower axes.  D is the
+// highest axis still to resolve.
+template <typename S, int D>
+__device__ double ghost_eval(const GhostArgs& a, int var, int c0, int c1, int c2) {
+#pragma clang fp contract(off)
+  const GridDesc& g = a.g;
+  const long long vo = (long long)var * g.fs;
+  if constexpr (D < 0) {
+    return (double)((const S*)a.f)[vo + lin(g, c0, c1, c2)];
+  } else {
+    int c[3] = {c0, c1, c2};
+    if (D >= g.ndim || (c[D] >= g.Mlo[D] && c[D] <= g.Mhi[D]))
+      return ghost_eval<S, D - 1>(a, var, c0, c1, c2);
+    const S* f = (const S*)a.f;
+    const S* snap = (const S*)a.snap;
+    const long long i = vo + lin(g, c[0], c[1], c[2]);
+    const int side = (c[D] > g.Mhi[D]) ? 1 : 0;
+    const int code = a.bc[var * 6 + 2 * D + side];
+    const int N = g.Mhi[D] - g.Mlo[D] + 1;
+    const int ixe = side ? g.Mhi[D] : g.Mlo[D];  // edge cell
+    auto at = [&](int cd) {
+      int e[3] = {c[0], c[1], c[2]};
+      e[D] = cd;
+      return ghost_eval<S, D - 1>(a, var, e[0], e[1], e[2]);
+    };
+    switch (code) {
+      case BC_PERIODIC:
+        return at(c[D] + (side ? -N : N));
+      case BC_SYMM:
+      case BC_SYMM0:
+        return at(side ? (2 * g.Mhi[D] + 1 - c[D]) : (2 * g.Mlo[D] - 1 - c[D]));
+      case BC_ASYMM:
+        return -at(side ? (2 * g.Mhi[D] + 1 - c[D]) : (2 * g.Mlo[D] - 1 - c[D]));
+      case BC_CONTCD4:  // mirror about the edge cell
+        return at(2 * ixe - c[D]);
+      case BC_FIXED:
+      case BC_FIXED1:
+      case BC_GRAD1:
+        if (snap) {
+          if (code == BC_GRAD1) return (double)snap[i] + at(ixe);
+          return (double)snap[i];
+        }
+        if (a.keep_fixed) return (double)f[i];  // fullgridini: file values
+        if (code == BC_FIXED) return at(ixe);
+        [[fallthrough]];
+      case BC_CONT1: {
+        const int k = abs(c[D] - ixe);
+        const int ixf = side ? ixe - 1 : ixe + 1;
+        return (double)(k + 1) * at(ixe) - (double)k * at(ixf);
+      }
+      default:  // cont
+        return at(ixe);
+    }
+  }
+}
+
 template <typename S>
 __device__ __forceinline__ S ghost_value(const GhostArgs& a, int var, int c0, int c1, int c2) {
+  return (S)ghost_eval<S, 2>(a, var, c0, c1, c2);
+}
+
+// Stored value of ghost cell c for the boundaries that keep one (fixed,
+// fixed1: the value; grad1: the difference to the edge cell of its axis,
+// src/vacgrid.t:476-507), from the filled field; 0 for other types.
+template <typename S>
+__device__ __forceinline__ S ghost_stored(const GhostArgs& a, int var, int c0, int c1, int c2) {
   const GridDesc& g = a.g;
   const S* f = (const S*)a.f;
-  const S* snap = (const S*)a.snap;
+  const long long vo = (long long)var * g.fs;
   int c[3] = {c0, c1, c2};
-  bool neg = false;
-  for (int iter = 0; iter < 8; iter++) {
-    int d = -1;
+  int d = -1;
 #pragma unroll
-    for (int dd = 2; dd >= 0; dd--) {
-      if (d < 0 && dd < g.ndim && (c[dd] < g.Mlo[dd] || c[dd] > g.Mhi[dd])) d = dd;
-    }
-    const long long i = lin(g, c[0], c[1], c[2]);
-    if (d < 0) return neg ? -f[(long long)var * g.fs + i] : f[(long long)var * g.fs + i];
-    const int side = (c[d] > g.Mhi[d]) ? 1 : 0;
-    const int code = a.bc[var * 6 + 2 * d + side];
-    const int N = g.Mhi[d] - g.Mlo[d] + 1;
-    if (code == BC_PERIODIC) {
-      c[d] += side ? -N : N;
-    } else if (code == BC_FIXED && snap) {
-      return neg ? -snap[(long long)var * g.fs + i] : snap[(long long)var * g.fs + i];
-    } else if (code == BC_FIXED && a.keep_fixed) {
-      return neg ? -f[(long long)var * g.fs + i] : f[(long long)var * g.fs + i];
-    } else if (code == BC_CONT || code == BC_FIXED) {
-      c[d] = side ? g.Mhi[d] : g.Mlo[d];
-    } else {  // symm, symm0, asymm: reflection about the boundary face
-      c[d] = side ? (2 * g.Mhi[d] + 1 - c[d]) : (2 * g.Mlo[d] - 1 - c[d]);
-      if (code == BC_ASYMM) neg = !neg;
-    }
+  for (int dd = 2; dd >= 0; dd--)
+    if (d < 0 && dd < g.ndim && (c[dd] < g.Mlo[dd] || c[dd] > g.Mhi[dd])) d = dd;
+  if (d < 0) return (S)0;
+  const int side = (c[d] > g.Mhi[d]) ? 1 : 0;
+  const int code = a.bc[var * 6 + 2 * d + side];
+  const S v = f[vo + lin(g, c[0], c[1], c[2])];
+  if (code == BC_FIXED || code == BC_FIXED1) return v;
+  if (code == BC_GRAD1) {
+    c[d] = side ? g.Mhi[d] : g.Mlo[d];
+    return (S)((double)v - (double)f[vo + lin(g, c[0], c[1], c[2])]);
   }
   return (S)0;
 }

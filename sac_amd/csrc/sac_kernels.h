@@ -68,6 +68,8 @@ void launch_axis(const AxisArgs& a, int axis, hipStream_t st);
 void launch_final(const FinalArgs& a, hipStream_t st);
 void launch_cmax(const CmaxArgs& a, hipStream_t st);
 void launch_ghost(const GhostArgs& a, hipStream_t st);
+// stored ghost values of the fixed/fixed1/grad1 boundaries of a.f (after a fill)
+void launch_ghost_store(const GhostArgs& a, void* store, hipStream_t st);
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 void launch_axpy4(int prec, long long n, void* out, const void* a0, double c0, const void* a1,
                   double c1, const void* a2, double c2, const void* a3, double c3,

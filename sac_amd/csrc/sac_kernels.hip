@@ -179,13 +179,10 @@ __global__ void __launch_bounds__(256) k_cmax(CmaxArgs a) {
 }
 
 // kernel 4: ghost-cell fill (see ghost_value)
-template <typename S>
-__global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
-  const GridDesc& g = a.g;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  // ghost shell = part A (ghost along axis 2 [0-based], any i0,i1)
-  //             + part B (mesh along 2, ghost along 1, any i0)
-  //             + part C (mesh along 2 and 1, ghost along 0)
+// cell t of the ghost shell = part A (ghost along axis 2 [0-based], any
+// i0,i1) + part B (mesh along 2, ghost along 1, any i0) + part C (mesh along
+// 2 and 1, ghost along 0); false past the shell
+__device__ __forceinline__ bool shell_cell(const GridDesc& g, long long t, int (&c)[3]) {
   const int G0 = g.G[0], G1 = g.G[1], G2 = g.G[2];
   const int N0 = g.Mhi[0] - g.Mlo[0] + 1;
   const int N1 = (g.ndim > 1) ? g.Mhi[1] - g.Mlo[1] + 1 : 1;
@@ -196,7 +193,6 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
   const long long nA = (long long)gh2 * G1 * G0;
   const long long nB = (long long)N2 * gh1 * G0;
   const long long nC = (long long)N2 * N1 * gh0;
-  int c[3];
   if (t < nA) {
     c[0] = (int)(t % G0);
     long long r = t / G0;
@@ -218,14 +214,33 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
     c[1] = g.Mlo[1] + (int)(r % N1);
     c[2] = g.Mlo[2] + (int)(r / N1);
   } else {
-    return;
+    return false;
   }
+  return true;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
+  const GridDesc& g = a.g;
+  int c[3];
+  if (!shell_cell(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
   const long long io = lin(g, c[0], c[1], c[2]);
   S* f = (S*)a.f;
   for (int v = 0; v < a.nvar; v++) {
     const S val = ghost_value<S>(a, v, c[0], c[1], c[2]);
     f[(long long)v * g.fs + io] = val;
   }
+}
+
+// stored values of the fixed / fixed1 / grad1 ghosts of the filled field a.f
+template <typename S>
+__global__ void __launch_bounds__(256) k_ghost_store(GhostArgs a, S* __restrict__ store) {
+  const GridDesc& g = a.g;
+  int c[3];
+  if (!shell_cell(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  const long long io = lin(g, c[0], c[1], c[2]);
+  for (int v = 0; v < a.nvar; v++)
+    store[(long long)v * g.fs + io] = ghost_stored<S>(a, v, c[0], c[1], c[2]);
 }
 
 // ---------------------------------------------------------------------
@@ -361,6 +376,19 @@ void launch_ghost(const GhostArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_ghost<float>, grid, dim3(bs), 0, st, a);
   else
     hipLaunchKernelGGL(k_ghost<double>, grid, dim3(bs), 0, st, a);
+}
+
+void launch_ghost_store(const GhostArgs& a, void* store, hipStream_t st) {
+  const GridDesc& g = a.g;
+  long long ncell = (long long)g.G[0] * g.G[1] * g.G[2];
+  long long n = ncell - mesh_cells(g);
+  if (n <= 0) return;
+  const int bs = 256;
+  const dim3 grid((unsigned)((n + bs - 1) / bs));
+  if (a.prec == PREC_MIXED)
+    hipLaunchKernelGGL(k_ghost_store<float>, grid, dim3(bs), 0, st, a, (float*)store);
+  else
+    hipLaunchKernelGGL(k_ghost_store<double>, grid, dim3(bs), 0, st, a, (double*)store);
 }
 
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t st) {

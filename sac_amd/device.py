@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libsac_amd.so")
 SLOT_W, SLOT_S1, SLOT_S2, SLOT_ACC = 0, 1, 2, 3
 K_FINAL, K_AXIS, K_GHOST, K_CMAX, K_COPY = 0, 1, 2, 3, 4
 K_AXIS1, K_AXIS2, K_AXIS3 = 5, 6, 7
-ADV = {"onestep": 1, "fourstep": 4}
+ADV = {"onestep": 1, "twostep": 2, "threestep": 3, "fourstep": 4, "sterck": 5, "jameson": 6}
 PREC = {"fp64": 0, "mixed": 1}
 
 EXPORTS = [

@@ -37,7 +37,8 @@ def test_getboundary_all_types(dev_mod, oracle_mod):
     """ghost fill of every boundary type against the sequential reference
     fill, on index-coded data (each cell value unique)"""
     for codes in ([1, 1, 3, 3, 2, 2], [3, 3, 1, 1, 3, 3], [4, 6, 2, 1, 3, 3],
-                  [6, 4, 3, 3, 5, 2]):
+                  [6, 4, 3, 3, 5, 2], [7, 8, 9, 10, 3, 3], [10, 9, 8, 7, 7, 8],
+                  [9, 9, 10, 10, 8, 7], [3, 3, 9, 8, 10, 4]):
         prob = problems.stratified(n=8, nxyz=(8, 7, 6))
         prob.typeB[:, :] = np.array(codes, dtype=np.int32)[None, :]
         # asymm on a different variable set than symm
@@ -201,3 +202,21 @@ def test_onestep(dev_mod, oracle_mod):
     got, gdts = _gpu_run(dev_mod, prob, 20, typeadvance="onestep")
     np.testing.assert_allclose(gdts, rdts, rtol=1e-12)
     assert_close_state(prob, got, ref, TOL, 6, "onestep")
+
+
+@pytest.mark.parametrize("adv", ["twostep", "threestep", "sterck", "jameson"])
+def test_integrators(dev_mod, oracle_mod, adv):
+    """the remaining Runge-Kutta integrators of src/vac.t:406-475 against the
+    oracle: 3-D stratified (free-running, well-conditioned dt) and a
+    courant-limited Brio-Wu tube"""
+    prob = problems.stratified(n=16, nxyz=(16, 12, 10))
+    ref, rdts, _ = oracle_run(prob, oracle_mod, 6, typeadvance=adv)
+    got, gdts = _gpu_run(dev_mod, prob, 6, typeadvance=adv)
+    np.testing.assert_allclose(gdts, rdts, rtol=1e-12)
+    assert_close_state(prob, got, ref, TOL, 8, adv)
+    prob = problems.brio_wu(n=200)
+    prob.courantpar = 0.05
+    ref, rdts, _ = oracle_run(prob, oracle_mod, 15, typeadvance=adv)
+    got, gdts = _gpu_run(dev_mod, prob, 15, typeadvance=adv)
+    np.testing.assert_allclose(gdts, rdts, rtol=1e-12)
+    assert_close_state(prob, got, ref, TOL, 6, adv)

@@ -688,3 +688,140 @@ def test_getboundary_semantics_1d(oracle_mod):
             w2[:, 0, 0, 2:-2] *= 3.0
             w3 = o.getboundary_on(w2)
             np.testing.assert_array_equal(w3[0, 0, 0, :2], exp_lo)  # restored, not re-copied
+
+
+# stability polynomials sum_k c_k z^k of the integrators (src/vac.t:406-475)
+# for a linear right-hand side: twostep is the midpoint rule, threestep is
+# Shu's SSP-RK3, fourstep and jameson are exact to z^4 for linear problems,
+# sterck's stage factors 0.12, 1/4, 1/2, 1 give 1/8 and 0.12/8 (0.12 is a
+# default-real literal there)
+_STAB = {
+    "onestep": [1.0, 1.0],
+    "twostep": [1.0, 1.0, 0.5],
+    "threestep": [1.0, 1.0, 0.5, 1.0 / 6.0],
+    "fourstep": [1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0],
+    "jameson": [1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0],
+    "sterck": [1.0, 1.0, 0.5, 1.0 / 8.0, float(np.float32(0.12)) / 8.0],
+}
+
+
+@pytest.mark.parametrize("adv", sorted(_STAB))
+def test_integrator_stability_polynomial(oracle_mod, adv):
+    """One step of a small perturbation of a uniform periodic MHD state
+    (no hyperdiffusion, so the right-hand side is linear in the perturbation
+    up to second-order terms) equals sum_k c_k (dt L)^k delta, with
+    (dt L)^k delta built by repeated right-hand-side evaluations.  The
+    polynomials differ at z^2..z^4 by 1e-3..1e-2 of the step at this dt, far
+    above the 1e-5 left by the nonlinear terms."""
+    prob, v = _blank(2, 3, (24, 20))
+    ne = v["nevolve"]
+    rng = np.random.default_rng(11)
+    u0 = prob.w.copy()
+    u0[0] = 1.0
+    u0[v["e"] - 1] = 1.5
+    u0[v["b0"]] = 0.6
+    u0[v["b0"] + 1] = -0.4
+    u0[v["m0"]] = 0.3
+    d = np.zeros_like(u0)
+    x, y = _coords(prob, 0), _coords(prob, 1)
+    for iw in range(ne):
+        a, b, ph = rng.uniform(-1, 1, 3)
+        d[iw] = 1e-7 * (a * np.sin(2 * np.pi * x + ph) + b * np.cos(2 * np.pi * (x + 2 * y)))
+    prob.w = u0 + d
+    o = make_oracle(prob, oracle_mod)
+    w = o.get_w()
+    base = o.getboundary_on(u0)
+    dt = 4e-3  # courant number ~0.3 on this grid
+
+    def dtL(vec):
+        s = base.copy()
+        s[:ne] = base[:ne] + vec[:ne]
+        s = o.getboundary_on(s)
+        r = o.advect1(dt, s, _zero_evolving(s, ne))
+        return r[:ne]
+
+    terms = [w[:ne] - base[:ne]]
+    for _ in range(len(_STAB[adv]) - 1):
+        terms.append(dtL(terms[-1]))
+    exp = sum(c * t for c, t in zip(_STAB[adv][1:], terms[1:]))
+    o.advance(dt, adv)
+    got = o.get_w()[:ne] - w[:ne]
+    for iw in range(ne):
+        scale = np.abs(mesh(prob, exp[iw])).max()
+        if scale == 0.0:
+            continue
+        np.testing.assert_allclose(mesh(prob, got[iw]), mesh(prob, exp[iw]), atol=1e-5 * scale,
+                                   rtol=0, err_msg=f"{adv} var {iw + 1}")
+
+
+def test_threestep_is_ssp_rk3_butcher(oracle_mod):
+    """advance('threestep') (Shu-Osher form) equals the Butcher form of SSP-RK3:
+    k1=R(w), k2=R(w+dt k1), k3=R(w+dt/4 (k1+k2)), w+dt (k1+k2+4k3)/6"""
+    prob = problems.stratified(n=10, nxyz=(10, 8, 8), amp=5e-2)
+    o = make_oracle(prob, oracle_mod)
+    w = o.get_w()
+    dt = 2e-3
+    ne = 8
+
+    def R(state):
+        return o.advect1(1.0, state, _zero_evolving(state, ne))
+
+    k1 = R(w)
+    s = w.copy(); s[:ne] = w[:ne] + dt * k1[:ne]; s = o.getboundary_on(s)
+    k2 = R(s)
+    s = w.copy(); s[:ne] = w[:ne] + dt / 4 * (k1[:ne] + k2[:ne]); s = o.getboundary_on(s)
+    k3 = R(s)
+    exp = w[:ne] + dt * (k1[:ne] + k2[:ne] + 4 * k3[:ne]) / 6
+    o.advance(dt, "threestep")
+    got = o.get_w()[:ne]
+    for iw in range(ne):
+        scale = np.abs(mesh(prob, exp[iw] - w[iw])).max() + 1e-300
+        np.testing.assert_allclose(mesh(prob, got[iw] - w[iw]), mesh(prob, exp[iw] - w[iw]),
+                                   atol=1e-11 * scale, rtol=0, err_msg=f"var {iw + 1}")
+
+
+def test_extrapolating_and_mirror_boundaries(oracle_mod):
+    """cont1, fixed1, grad1 and contCD4 (src/vacgrid.t:260-290,376-407,476-507)
+    on a 1-D tube with hand-computed ghosts: mesh values q_i = 3 + 2 i
+    (i = 1..N) plus a bump at the edge cells so that mirror and
+    extrapolation differ"""
+    from sac_amd import problems as P
+    n = 8
+    prob, v = _blank(1, 2, (n,), bc="cont")
+    codes = {1: P.BC["cont1"], 2: P.BC["fixed1"], 3: P.BC["grad1"], 4: P.BC["contCD4"]}
+    for iw, c in codes.items():
+        prob.typeB[iw - 1, :] = c
+    q = 3.0 + 2.0 * np.arange(1, n + 1)
+    q[0] += 0.5     # first mesh cell
+    q[-1] -= 0.25   # last mesh cell
+    for iw in codes:
+        prob.w[iw - 1, 0, 0, 2:-2] = q
+    o = make_oracle(prob, oracle_mod)  # the first getboundary stores fixed1/grad1
+    w = o.get_w()[:, 0, 0, :]
+    # linear extrapolation from the edge and the inner edge
+    lo = [q[0] + 2 * (q[0] - q[1]), q[0] + (q[0] - q[1])]
+    hi = [q[-1] + (q[-1] - q[-2]), q[-1] + 2 * (q[-1] - q[-2])]
+    for iw in (1, 2, 3):
+        np.testing.assert_allclose(w[iw - 1, :2], lo, rtol=1e-15)
+        np.testing.assert_allclose(w[iw - 1, -2:], hi, rtol=1e-15)
+    # contCD4: ghost i-1 <- cell i+1, ghost i-2 <- cell i+2 (edge cell i)
+    np.testing.assert_array_equal(w[3, :2], [q[2], q[1]])
+    np.testing.assert_array_equal(w[3, -2:], [q[-2], q[-3]])
+    # second fill after the mesh changed: cont1 extrapolates again, fixed1
+    # restores the stored values, grad1 keeps the stored differences to the
+    # new edge values
+    w2 = o.get_w()
+    q2 = q * 1.5 - 1.0
+    for iw in codes:
+        w2[iw - 1, 0, 0, 2:-2] = q2
+    r = o.getboundary_on(w2)[:, 0, 0, :]
+    lo2 = [q2[0] + 2 * (q2[0] - q2[1]), q2[0] + (q2[0] - q2[1])]
+    hi2 = [q2[-1] + (q2[-1] - q2[-2]), q2[-1] + 2 * (q2[-1] - q2[-2])]
+    np.testing.assert_allclose(r[0, :2], lo2, rtol=1e-15)
+    np.testing.assert_allclose(r[0, -2:], hi2, rtol=1e-15)
+    np.testing.assert_array_equal(r[1, :2], w[1, :2])
+    np.testing.assert_array_equal(r[1, -2:], w[1, -2:])
+    np.testing.assert_allclose(r[2, :2], [q2[0] + (lo[0] - q[0]), q2[0] + (lo[1] - q[0])], rtol=1e-15)
+    np.testing.assert_allclose(r[2, -2:], [q2[-1] + (hi[0] - q[-1]), q2[-1] + (hi[1] - q[-1])],
+                               rtol=1e-15)
+    np.testing.assert_array_equal(r[3, :2], [q2[2], q2[1]])
