@@ -82,6 +82,9 @@ struct Dev {
   double dtgrav = kBigDouble;
   double dxmin[3] = {0, 0, 0};
   bool any_fixed = false;
+  // boundary fills with two sources: cont1 always, fixed1/grad1 until
+  // their values are stored (evolving fields, background)
+  bool cont1_e = false, ext1_e = false, cont1_b = false, ext1_b = false;
   bool initialized = false;  // getboundary's 'initialized' (src/vacgrid.t:209)
   // device buffers (evolving fields, R, T and snap in the storage precision)
   void* reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
@@ -164,6 +167,7 @@ void ghost_fill_evolving(Dev& d, int slot) {
   a.nvar = d.ne;
   a.snap = d.initialized ? d.snap : nullptr;
   a.keep_fixed = (!d.initialized && d.fullgridini) ? 1 : 0;
+  a.general = d.cont1_e || (d.ext1_e && !a.snap && !a.keep_fixed);
   hipEvent_t e0{};
   t_begin(d, SAC_K_GHOST, &e0);
   sac::launch_ghost(a, d.stream);
@@ -468,10 +472,16 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   for (int iw = 0; iw < nw; iw++)
     for (int iB = 0; iB < 2 * ndim; iB++) {
       int c = typeB[iw * 2 * ndim + iB];
-      if (iw < ne)
+      const bool c1 = c == SAC_BC_CONT1, x1 = c == SAC_BC_FIXED1 || c == SAC_BC_GRAD1;
+      if (iw < ne) {
         bce[(size_t)iw * 6 + iB] = c;
-      else
+        d->cont1_e |= c1;
+        d->ext1_e |= x1;
+      } else {
         bcb[(size_t)(iw - ne) * 6 + iB] = c;
+        d->cont1_b |= c1;
+        d->ext1_b |= x1;
+      }
     }
   if (hipMalloc((void**)&d->bc_e, sizeof(int) * bce.size()) != hipSuccess ||
       hipMalloc((void**)&d->bc_b, sizeof(int) * bcb.size()) != hipSuccess)
@@ -576,6 +586,7 @@ int sac_dev_getboundary(sac_handle h) {
     a.nvar = d.nbg;
     a.snap = nullptr;
     a.keep_fixed = d.fullgridini ? 1 : 0;
+    a.general = d.cont1_b || (d.ext1_b && !a.keep_fixed);
     sac::launch_ghost(a, d.stream);
     ghost_fill_evolving(d, SAC_SLOT_W);
     if (d.any_fixed) {

@@ -708,9 +708,60 @@ __device__ double ghost_eval(const GhostArgs& a, int var, int c0, int c1, int c2
   }
 }
 
-template <typename S>
+// The common path of ghost_eval, iterative: single-source fills move the
+// cell into the mesh axis by axis, keeping result = sgn * value + add; the
+// two-source extrapolations (cont1, and fixed1/grad1 before their values
+// are stored) hand the rest to ghost_eval, compiled only into the GENERAL
+// kernel (the launcher picks it when such a fill occurs).
+template <typename S, bool GENERAL>
 __device__ __forceinline__ S ghost_value(const GhostArgs& a, int var, int c0, int c1, int c2) {
-  return (S)ghost_eval<S, 2>(a, var, c0, c1, c2);
+  const GridDesc& g = a.g;
+  const S* f = (const S*)a.f;
+  const S* snap = (const S*)a.snap;
+  const long long vo = (long long)var * g.fs;
+  int c[3] = {c0, c1, c2};
+  double sgn = 1.0, add = 0.0;
+  for (int iter = 0; iter < 3; iter++) {
+    int d = -1;
+#pragma unroll
+    for (int dd = 2; dd >= 0; dd--) {
+      if (d < 0 && dd < g.ndim && (c[dd] < g.Mlo[dd] || c[dd] > g.Mhi[dd])) d = dd;
+    }
+    const long long i = vo + lin(g, c[0], c[1], c[2]);
+    if (d < 0) break;
+    const int side = (c[d] > g.Mhi[d]) ? 1 : 0;
+    const int code = a.bc[var * 6 + 2 * d + side];
+    const int N = g.Mhi[d] - g.Mlo[d] + 1;
+    const int ixe = side ? g.Mhi[d] : g.Mlo[d];
+    const bool stored = code == BC_FIXED || code == BC_FIXED1 || code == BC_GRAD1;
+    if (code == BC_PERIODIC) {
+      c[d] += side ? -N : N;
+    } else if (stored && snap) {
+      if (code != BC_GRAD1) return (S)(sgn * (double)snap[i] + add);
+      add += sgn * (double)snap[i];
+      c[d] = ixe;
+    } else if (stored && a.keep_fixed) {
+      return (S)(sgn * (double)f[i] + add);
+    } else if (code == BC_CONT || code == BC_FIXED) {
+      c[d] = ixe;
+    } else if (code == BC_CONTCD4) {
+      c[d] = 2 * ixe - c[d];
+    } else if (code == BC_SYMM || code == BC_SYMM0 || code == BC_ASYMM) {
+      c[d] = side ? (2 * g.Mhi[d] + 1 - c[d]) : (2 * g.Mlo[d] - 1 - c[d]);
+      if (code == BC_ASYMM) sgn = -sgn;
+    } else {  // two-source extrapolation: the general evaluator from here
+      if constexpr (GENERAL) {
+        double v;
+        if (d == 2) v = ghost_eval<S, 2>(a, var, c[0], c[1], c[2]);
+        else if (d == 1) v = ghost_eval<S, 1>(a, var, c[0], c[1], c[2]);
+        else v = ghost_eval<S, 0>(a, var, c[0], c[1], c[2]);
+        return (S)(sgn * v + add);
+      } else {
+        return (S)0;  // not reached: the launcher selects the GENERAL kernel
+      }
+    }
+  }
+  return (S)(sgn * (double)f[vo + lin(g, c[0], c[1], c[2])] + add);
 }
 
 // Stored value of ghost cell c for the boundaries that keep one (fixed,

@@ -62,6 +62,7 @@ struct GhostArgs {
   const int* bc;       // [nvar][6] boundary codes
   int nvar;
   int keep_fixed;      // initial fill with fullgridini: keep file values
+  int general;         // some fill has two sources (cont1; fixed1/grad1 before storage)
 };
 
 void launch_axis(const AxisArgs& a, int axis, hipStream_t st);

@@ -219,7 +219,7 @@ __device__ __forceinline__ bool shell_cell(const GridDesc& g, long long t, int (
   return true;
 }
 
-template <typename S>
+template <typename S, bool GENERAL>
 __global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
   const GridDesc& g = a.g;
   int c[3];
@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a) {
   const long long io = lin(g, c[0], c[1], c[2]);
   S* f = (S*)a.f;
   for (int v = 0; v < a.nvar; v++) {
-    const S val = ghost_value<S>(a, v, c[0], c[1], c[2]);
+    const S val = ghost_value<S, GENERAL>(a, v, c[0], c[1], c[2]);
     f[(long long)v * g.fs + io] = val;
   }
 }
@@ -372,10 +372,17 @@ void launch_ghost(const GhostArgs& a, hipStream_t st) {
   if (n <= 0) return;
   const int bs = 256;
   const dim3 grid((unsigned)((n + bs - 1) / bs));
-  if (a.prec == PREC_MIXED)
-    hipLaunchKernelGGL(k_ghost<float>, grid, dim3(bs), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_ghost<double>, grid, dim3(bs), 0, st, a);
+  if (a.prec == PREC_MIXED) {
+    if (a.general)
+      hipLaunchKernelGGL((k_ghost<float, true>), grid, dim3(bs), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_ghost<float, false>), grid, dim3(bs), 0, st, a);
+  } else {
+    if (a.general)
+      hipLaunchKernelGGL((k_ghost<double, true>), grid, dim3(bs), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_ghost<double, false>), grid, dim3(bs), 0, st, a);
+  }
 }
 
 void launch_ghost_store(const GhostArgs& a, void* store, hipStream_t st) {
