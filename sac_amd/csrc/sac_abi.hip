@@ -207,6 +207,7 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   x.ctot_src = red_of(d, wct);
   x.red_visc = red_visc(d);
   const bool learned = d.cl && d.cl->on;
+  int skip_last = 0;
   for (int k = 0; k < d.ndim; k++) {
     // axis 1 (rows of a plane): neighbour rows in one XCD's L2.  Axes 0 and 2
     // keep the hardware order, whose round-robin already places the
@@ -222,9 +223,10 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
       t_end(d, SAC_K_CLOSURE, ei);
       x.RF = d.cl->RF[k];
     }
+    x.cross_last = (k == d.ndim - 1) ? 1 : 0;
     hipEvent_t e0{};
     t_begin(d, SAC_K_AXIS, &e0);
-    sac::launch_axis(x, k, d.stream);
+    skip_last = sac::launch_axis(x, k, d.stream);
     t_end(d, SAC_K_AXIS, e0, 5 + k);
   }
   sac::launch_zero_u64(red_of(d, out), sac::kRedSlots, d.stream);
@@ -250,6 +252,7 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   s.visc_on = d.visc_on;
   s.red_out = red_of(d, out);
   s.xcd = 1;
+  s.skip_last = skip_last;
   hipEvent_t e0{};
   t_begin(d, SAC_K_FINAL, &e0);
   sac::launch_final(s, d.stream);

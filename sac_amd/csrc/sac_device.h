@@ -580,6 +580,7 @@ __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g
   if (a.visc_on && NDIM > 1) {
 #pragma unroll
     for (int l = 0; l < NDIM; l++) {
+      if (l == NDIM - 1 && a.skip_last) continue;  // added by the line kernel of axis l
       if ((c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l])) continue;
       const long long sl = g.st[l];
       const C h = (C)(0.5 / a.dx[l][c[l]]);

@@ -21,6 +21,7 @@ struct AxisArgs {
   unsigned long long* red_visc;        // maxviscoef accumulator
   int xcd;                             // XCD count for block renumbering (1 = off)
   const float* RF;                     // learned face ratios of axis K, or null (exact)
+  int cross_last;                      // last axis: add the cross-term derivatives along it
 };
 
 struct FinalArgs {
@@ -42,6 +43,7 @@ struct FinalArgs {
   int visc_on;
   unsigned long long* red_out;  // cmax reductions of out
   int xcd;
+  int skip_last;                // cross-term derivatives along the last axis already in R
 };
 
 struct CmaxArgs {
@@ -65,7 +67,9 @@ struct GhostArgs {
   int general;         // some fill has two sources (cont1; fixed1/grad1 before storage)
 };
 
-void launch_axis(const AxisArgs& a, int axis, hipStream_t st);
+// returns 1 if the cross-term derivatives along `axis` were added (requested
+// by a.cross_last and done by the line-tile kernel), else 0
+int launch_axis(const AxisArgs& a, int axis, hipStream_t st);
 void launch_final(const FinalArgs& a, hipStream_t st);
 void launch_cmax(const CmaxArgs& a, hipStream_t st);
 void launch_ghost(const GhostArgs& a, hipStream_t st);

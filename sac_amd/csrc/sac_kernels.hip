@@ -343,7 +343,7 @@ static long long mesh_cells(const GridDesc& g) {
   return n;
 }
 
-void launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
+int launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
   const GridDesc& g = a.g;
   long long n = 1;
   for (int d = 0; d < 3; d++) {
@@ -355,6 +355,8 @@ void launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
     n *= e;
   }
   dispatch(a.prec, g.ndim, g.ndir, 0, axis, &a, st, n);
+  const bool line = SAC_LINE_TILES && g.ndim > 1 && !a.RF;
+  return (line && a.cross_last && a.visc_on && axis == g.ndim - 1) ? 1 : 0;
 }
 
 void launch_final(const FinalArgs& a, hipStream_t st) {

@@ -139,18 +139,18 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
     return (c < 0) ? 0 : ((c > GK - 1) ? GK - 1 : c);
   };
   auto cell_index = [&](int j) { return base + (long long)cell_clamped(j) * sk; };
-  // Software pipeline: the point, the cell width and (K > 0) the rate
-  // accumulator of chunk j+1 are loaded while chunk j is evaluated.  The
-  // cell's loads are issued first so that waiting for them never waits for
-  // the point loads behind them.
+  // Software pipeline: the point of chunk j+1 is loaded while chunk j is
+  // evaluated.  The cell's own loads (rate accumulator for K > 0, cell
+  // width, and on the last axis the cross-term factors) are issued at the
+  // start of chunk j, before the next point's loads, so that waiting for
+  // them never waits for those; their latency hides behind the point phase.
+  //
+  // Last axis: the cross-term factors T_{k,K} (k < K) at the cell's
+  // neighbours along K, in the order Tm(c+1), Tm(c-1), Tb(c+1), Tb(c-1).
+  constexpr bool kCross = (K == NDIM - 1 && NDIM > 1);
+  constexpr int NTC = kCross ? 4 * K : 1;
+  const bool do_cross = kCross && a.cross_last && a.visc_on;
   Pt<NDIR, C> nxt;
-  C Rn[NE];
-  double dxn = a.dx[K][cell_clamped(0)];
-  if (K > 0) {
-    const long long ic = cell_index(0);
-#pragma unroll
-    for (int n = 0; n < NE; n++) Rn[n] = (C)Racc[n * fs + ic];
-  }
   load_pt<NDIR, S, C>(Wf, Bg, fs, point_index(0), nxt);
 
   int rb = 0;  // ring row of the chunk's first point
@@ -160,18 +160,27 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
     if (r0 >= RR) r0 -= RR;
     const Pt<NDIR, C> pt = nxt;
     C Rold[NE];
+    const long long icell = cell_index(j);
 #pragma unroll
-    for (int n = 0; n < NE; n++) Rold[n] = (K > 0) ? Rn[n] : (C)0;
-    const double dxd = dxn;
-    if (j + 1 < nch) {
-      dxn = a.dx[K][cell_clamped(j + 1)];
-      if (K > 0) {
-        const long long ic = cell_index(j + 1);
+    for (int n = 0; n < NE; n++) Rold[n] = (K > 0) ? (C)Racc[n * fs + icell] : (C)0;
+    const double dxd = a.dx[K][cell_clamped(j)];
+    C Tc[NTC];
+    if constexpr (kCross) {
+      if (do_cross) {
+        const int cc = cell_clamped(j);
+        const long long ipc = base + (long long)((cc + 1 <= GK - 1) ? cc + 1 : GK - 1) * sk;
+        const long long imc = base + (long long)((cc - 1 >= 0) ? cc - 1 : 0) * sk;
 #pragma unroll
-        for (int n = 0; n < NE; n++) Rn[n] = (C)Racc[n * fs + ic];
+        for (int k = 0; k < K; k++) {
+          const long long xi = (long long)cross_index(NDIM, k, K) * fs;
+          Tc[4 * k + 0] = (C)Tout[xi + ipc];
+          Tc[4 * k + 1] = (C)Tout[xi + imc];
+          Tc[4 * k + 2] = (C)Tout[xi + fs + ipc];
+          Tc[4 * k + 3] = (C)Tout[xi + fs + imc];
+        }
       }
-      load_pt<NDIR, S, C>(Wf, Bg, fs, point_index(j + 1), nxt);
     }
+    if (j + 1 < nch) load_pt<NDIR, S, C>(Wf, Bg, fs, point_index(j + 1), nxt);
     // ---- the new point: evaluate what depends on it alone ----
     {
       C F[NE], v[NDIM];
@@ -220,7 +229,6 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
         return (r < 0) ? r + RR : r;
       };
       const C dxk = (C)dxd, idx = (C)(1.0 / dxd);
-      const long long icell = cell_index(j);
       C R[NE];
       {
         const bool fluxc = c >= MloK + 2 && c <= MhiK - 2;
@@ -247,6 +255,33 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
         R[1 + NDIR] += re;
 #pragma unroll
         for (int n = 0; n < NE; n++) R[n] = fluxc ? R[n] : (C)0;
+      }
+      // The last axis runs after the others have written their cross-term
+      // factors T_{k,K}: their derivatives along K (central, zero at a
+      // non-periodic edge of K, T = 0 outside the mesh) are added here from
+      // the ring instead of in the combining kernel:
+      //   m_k += d_K(rho_t Tm_{k,K}), e += d_K(m_k Tm_{k,K}),
+      //   b_k -= d_K(Tb_{k,K}),       e -= d_K(b_k Tb_{k,K})
+      if constexpr (kCross) {
+        const bool z1 = (c == MloK && !g.per_lo[K]) || (c == MhiK && !g.per_hi[K]);
+        if (do_cross && valid && !z1) {
+          const int rp1 = row(1), rm1 = row(-1);
+          const C h = (C)(0.5 / dxd);
+          const bool inP = c + 1 <= MhiK, inM = c - 1 >= MloK;
+          const C rtp = L[at<K>(SL::RT, rp1, iw)], rtm = L[at<K>(SL::RT, rm1, iw)];
+#pragma unroll
+          for (int k = 0; k < K; k++) {
+            const C Tmp = inP ? Tc[4 * k + 0] : (C)0, Tmm = inM ? Tc[4 * k + 1] : (C)0;
+            const C Tbp = inP ? Tc[4 * k + 2] : (C)0, Tbm = inM ? Tc[4 * k + 3] : (C)0;
+            const C mkp = L[at<K>(SL::M0 + k, rp1, iw)], mkm = L[at<K>(SL::M0 + k, rm1, iw)];
+            const C bkp = L[at<K>(SL::Q0 + 2 + NDIM + k, rp1, iw)];
+            const C bkm = L[at<K>(SL::Q0 + 2 + NDIM + k, rm1, iw)];
+            R[1 + k] += (rtp * Tmp - rtm * Tmm) * h;
+            R[1 + NDIR] += (mkp * Tmp - mkm * Tmm) * h;
+            R[2 + NDIR + k] -= (Tbp - Tbm) * h;
+            R[1 + NDIR] -= (Tbp * bkp - Tbm * bkm) * h;
+          }
+        }
       }
       if (a.visc_on) {
 #pragma clang fp contract(off)
