@@ -117,7 +117,8 @@ static int line_segment(const GridDesc& g) {
 template <int NDIM, int NDIR, int K, typename S, typename C>
 static void launch_line(const AxisArgs& A, hipStream_t st) {
   const int seg = line_segment<NDIM, K>(A.g);
-  const long long nb = line::nblocks<NDIM, K>(A.g, seg);
+  long long nb = line::nblocks<NDIM, K>(A.g, seg);
+  nb = (nb + 15) / 16 * 16;  // tile pairing over the XCDs (see line_tile)
   if (nb > 0)
     hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C>), dim3((unsigned)nb), dim3(line::NT), 0, st, A,
                        seg);

@@ -69,11 +69,20 @@ struct Axes {
   static constexpr int B = (NDIM == 3) ? ((K == 2) ? 1 : 2) : -1;
 };
 
+// First grid index of the line tiles along A: tiles along x start at the
+// row start, so that a tile's 16 values of a row fill whole 128-byte lines
+// (rows are padded to a multiple of 16 elements) and no two tiles share a
+// line; the ghost columns of the first and last tile stay idle.
+template <int A>
+__host__ __device__ inline int tile_origin(const GridDesc& g) {
+  return (A == 0) ? 0 : g.Mlo[A];
+}
+
 // blocks of a launch for segments of `seg` cells along K
 template <int NDIM, int K>
 __host__ __device__ inline long long nblocks(const GridDesc& g, int seg) {
   constexpr int A = Axes<NDIM, K>::A, B = Axes<NDIM, K>::B;
-  const long long nA = (g.Mhi[A] - g.Mlo[A] + 1 + W - 1) / W;
+  const long long nA = (g.Mhi[A] - tile_origin<A>(g) + 1 + W - 1) / W;
   const long long nB = (B >= 0) ? (g.Mhi[B] - g.Mlo[B] + 1) : 1;
   const long long nK = (g.Mhi[K] - g.Mlo[K] + 1 + seg - 1) / seg;
   return nA * nB * nK;
@@ -96,9 +105,19 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
   const int ik = (K == 0) ? (tid % CH) : (tid / W);  // position in the chunk
   const int iw = (K == 0) ? (tid / CH) : (tid % W);  // line of the block
 
-  // block -> (line tile along A, line along B, segment along K)
+  // block -> (line tile along A, line along B, segment along K).  Workgroups
+  // are dealt to the 8 XCDs in turn (block b to XCD b % 8); for tiles along x
+  // the logical index pairs tiles 2m and 2m+1, whose rows share 128-byte
+  // lines when a tile row is shorter than a line (FP32 storage), on one XCD
+  // and in consecutive rounds, so the line is fetched into one L2.  The
+  // launch is padded to a multiple of 16 blocks; the padding blocks exit.
   long long b = blockIdx.x;
-  const int nA = (g.Mhi[A] - g.Mlo[A] + 1 + W - 1) / W;
+  if constexpr (A == 0) {
+    const long long x = b % 8, r = b / 8;
+    b = 2 * (x + 8 * (r / 2)) + (r % 2);
+  }
+  if (b >= nblocks<NDIM, K>(g, seg)) return;
+  const int nA = (g.Mhi[A] - tile_origin<A>(g) + 1 + W - 1) / W;
   const int bA = (int)(b % nA);
   b /= nA;
   int cB = 0;
@@ -108,9 +127,11 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
     b /= nB;
   }
   const int sIdx = (int)b;
-  int cA = g.Mlo[A] + bA * W + iw;
-  const bool line_ok = cA <= g.Mhi[A];
-  if (!line_ok) cA = g.Mhi[A];  // computed on an in-grid line, never stored
+  int cA = tile_origin<A>(g) + bA * W + iw;
+  const bool line_ok = cA >= g.Mlo[A] && cA <= g.Mhi[A];
+  // an idle line is computed on an in-grid line and never stored
+  if (cA > g.Mhi[A]) cA = g.Mhi[A];
+  if (cA < g.Mlo[A]) cA = g.Mlo[A];
   int c3[3] = {0, 0, 0};
   c3[A] = cA;
   if constexpr (B >= 0) c3[B] = cB;
@@ -118,8 +139,12 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
 
   const int cs = MloK + sIdx * seg;                      // first cell of the segment
   const int ce = (cs + seg - 1 < MhiK) ? cs + seg - 1 : MhiK;
-  const int ps = cs - HALO;                              // first point of the ring
-  const int nch = (ce - cs + 1 + 2 * HALO + CH - 1) / CH;
+  // first point of the ring: three before the first cell; along x rounded
+  // down to a multiple of CH so that every chunk reads whole 128-byte lines
+  // (the extra points only feed cells before the segment, which are idle)
+  int ps = cs - HALO;
+  if (K == 0) ps = (ps >= 0) ? ps / CH * CH : -((-ps + CH - 1) / CH) * CH;
+  const int nch = (ce + HALO - ps + 1 + CH - 1) / CH;
 
   const C ctk = a.visc_on ? (C)red_get(a.ctot_src, K) : (C)0;
   S* __restrict__ Racc = (S*)a.R;
