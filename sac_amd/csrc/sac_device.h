@@ -564,24 +564,31 @@ __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g
   for (int n = 0; n < NE; n++) R[n] = (C)RR[n * fs + ic];
   // gravity on the perturbed density (addsource_grav, src/vacusr.gravity.t:46-78)
   {
+    // loads first, then selects (no branch between the loads)
     const C rho = (C)W[ic];
     const C inv = (C)recip<C>((double)W[ic] + B[fs + ic]);
+    C md[NDIM];
+#pragma unroll
+    for (int d = 0; d < NDIM; d++) md[d] = (C)W[(1 + d) * fs + ic];
     C rg = 0;
 #pragma unroll
     for (int d = 0; d < NDIM; d++) {
-      if (a.grav_on[d]) {
-        const C gd = (C)a.grav[d];
-        R[1 + d] += gd * rho;
-        rg += gd * rho * (C)W[(1 + d) * fs + ic] * inv;
-      }
+      const bool on = a.grav_on[d] != 0;
+      const C gd = (C)a.grav[d];
+      R[1 + d] = on ? R[1 + d] + gd * rho : R[1 + d];
+      rg = on ? rg + gd * rho * md[d] * inv : rg;
     }
     R[1 + NDIR] += rg;
   }
   if (a.visc_on && NDIM > 1) {
+    // Branch-free, so that every load of the cell can be issued at once: the
+    // neighbours c +- e_l of a mesh cell are in the grid; T outside the mesh
+    // (never written) and the edge cells' terms are replaced by zero with
+    // selects, never multiplied in.
 #pragma unroll
     for (int l = 0; l < NDIM; l++) {
       if (l == NDIM - 1 && a.skip_last) continue;  // added by the line kernel of axis l
-      if ((c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l])) continue;
+      const bool edge = (c[l] == g.Mlo[l] && !g.per_lo[l]) || (c[l] == g.Mhi[l] && !g.per_hi[l]);
       const long long sl = g.st[l];
       const C h = (C)(0.5 / a.dx[l][c[l]]);
       const bool inP = c[l] + 1 <= g.Mhi[l], inM = c[l] - 1 >= g.Mlo[l];
@@ -591,14 +598,20 @@ __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g
       for (int k = 0; k < NDIM; k++) {
         if (k == l) continue;
         const long long xi = (long long)cross_index(NDIM, k, l) * fs;
-        const C Tmp = inP ? (C)TT[xi + ip] : (C)0, Tmm = inM ? (C)TT[xi + im] : (C)0;
-        const C Tbp = inP ? (C)TT[xi + fs + ip] : (C)0, Tbm = inM ? (C)TT[xi + fs + im] : (C)0;
+        const C Tmp0 = (C)TT[xi + ip], Tmm0 = (C)TT[xi + im];
+        const C Tbp0 = (C)TT[xi + fs + ip], Tbm0 = (C)TT[xi + fs + im];
+        const C Tmp = inP ? Tmp0 : (C)0, Tmm = inM ? Tmm0 : (C)0;
+        const C Tbp = inP ? Tbp0 : (C)0, Tbm = inM ? Tbm0 : (C)0;
         const C mkp = (C)W[(1 + k) * fs + ip], mkm = (C)W[(1 + k) * fs + im];
         const C bkp = (C)W[(2 + NDIR + k) * fs + ip], bkm = (C)W[(2 + NDIR + k) * fs + im];
-        R[1 + k] += (rtp * Tmp - rtm * Tmm) * h;
-        R[1 + NDIR] += (mkp * Tmp - mkm * Tmm) * h;
-        R[2 + NDIR + k] -= (Tbp - Tbm) * h;
-        R[1 + NDIR] -= (Tbp * bkp - Tbm * bkm) * h;
+        const C dm = (rtp * Tmp - rtm * Tmm) * h;
+        const C de1 = (mkp * Tmp - mkm * Tmm) * h;
+        const C db = (Tbp - Tbm) * h;
+        const C de2 = (Tbp * bkp - Tbm * bkm) * h;
+        R[1 + k] += edge ? (C)0 : dm;
+        R[1 + NDIR] += edge ? (C)0 : de1;
+        R[2 + NDIR + k] -= edge ? (C)0 : db;
+        R[1 + NDIR] -= edge ? (C)0 : de2;
       }
     }
   }
