@@ -193,6 +193,7 @@ def main():
             "after_steps": args.steps + args.warmup, "finite": finm,
             "roofline": {"achieved": round(ach_m, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach_m / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic("mixed", n),
                          "bytes_per_cell_stage": BYTES_PER_CELL_STAGE["mixed"]}}}
     if not args.no_batch:
         out.setdefault("forms", {})["batched_tubes"] = batched_leg(args.tubes)
