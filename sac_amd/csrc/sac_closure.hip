@@ -52,6 +52,7 @@ __device__ __forceinline__ bool features(const GridDesc& g, const double* __rest
   const double cs2 = gamma * (pth + (gamma - 1.0) * (p0.eb - 0.5 * sbg2)) * inv;
   if (!(cs2 > 0.0)) return false;
   const double cs = sqrt(cs2), bsc = cs * sqrt(rt);
+  const double ics = 1.0 / cs, ibsc = 1.0 / bsc;
   constexpr int KB = (K + 1) % NDIR;
   const double v0 = p0.m[K] * inv, b0 = p0.b[KB];
   float smax = 0.f;
@@ -61,10 +62,10 @@ __device__ __forceinline__ bool features(const GridDesc& g, const double* __rest
     const double rho = W[i];
     const double v = W[(1 + K) * fs + i] / (rho + B[fs + i]);
     const double b = W[(2 + NDIR + KB) * fs + i];
-    const float xr = (float)((rho - p0.rho) * inv), xv = (float)((v - v0) / cs);
+    const float xr = (float)((rho - p0.rho) * inv), xv = (float)((v - v0) * ics);
     x[s] = xr;
     x[7 + s] = xv;
-    x[14 + s] = (float)((b - b0) / bsc);
+    x[14 + s] = (float)((b - b0) * ibsc);
     smax = fmaxf(smax, fmaxf(fabsf(xr), fabsf(xv)));
   }
   sensor = smax;
@@ -246,7 +247,19 @@ __global__ void k_to_bf16(const float* P32, uint16_t* P16) {
 }
 
 // ---------------------------------------------------------------------
-// inference: four wavefronts per block, one face per lane
+// inference: four wavefronts per block, one face per lane; the network runs
+// with its activations in registers (mlp::forward_regs), LDS only carries
+// the features into the K-major operand layout
+
+// setnu variable of output slot o on axis K (-1: slot unused)
+template <int NDIM, int K>
+__device__ __forceinline__ int var_of_slot(int o) {
+  int v = -1;
+#pragma unroll
+  for (int p = 0; p < 2 * NDIM + 1; p++)
+    if (slot_of<NDIM, K>(p) == o) v = p;
+  return v;
+}
 
 // mode 1: network ratios; mode 2: the exact ratios, in FP32, through the
 // same storage (a check of the plumbing: the run then follows the exact one
@@ -258,8 +271,6 @@ __global__ void __launch_bounds__(256) k_infer(GridDesc g, const double* w, cons
   using namespace mlp;
   constexpr int NP = 2 * NDIM + 1;
   __shared__ __attribute__((aligned(16))) uint16_t xs[4][WAVE * IN];
-  __shared__ __attribute__((aligned(16))) uint16_t ha[4][WAVE * H], hb[4][WAVE * H];
-  __shared__ __attribute__((aligned(16))) float os[4][WAVE * OUT];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // faces c+1/2 for c = Mlo-1 .. Mhi along K, mesh along the other axes
   int lo[3], ext[3];
@@ -278,22 +289,36 @@ __global__ void __launch_bounds__(256) k_infer(GridDesc g, const double* w, cons
   bool ok = false;
   if (active) ok = features<NDIM, NDIR, K>(g, w, bg, gamma, c, x, s);
   const bool use = ok && s >= kFlat && s < thr;
+  const long long ic = lin(g, c[0], c[1], c[2]);
+  if (mode == 2) {
+    if (active) {
+      float ex[7];
+      if (use) exact_ratios<NDIM, NDIR, K>(g, w, bg, c, ex);
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+        RF[(long long)p * g.fs + ic] = use ? fminf(fmaxf(6.f * ex[slot_of<NDIM, K>(p)], 0.f), 6.f) : -1.f;
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < IN; k++) xs[wv][lane * IN + k] = use ? f2bf(x[k]) : (uint16_t)0;
   __syncthreads();
-  forward(P16, P32, xs[wv], ha[wv], hb[wv], ha[wv], os[wv], lane);
-  if (active) {
-    const long long ic = lin(g, c[0], c[1], c[2]);
-    float ex[7];
-    if (mode == 2 && use) exact_ratios<NDIM, NDIR, K>(g, w, bg, c, ex);
+  f32x4 out[WAVE / 16];
+  forward_regs(P16, P32, xs[wv], out, lane);
+  // lane (r, q) holds outputs 4q..4q+3 of samples 16nb + r: each writes those
+  // of its samples, with the sample's cell index and flags from its lane
+  const int r = lane & 15, q = lane >> 4;
 #pragma unroll
-    for (int p = 0; p < NP; p++) {
-      float r = -1.f;
-      if (use) {
-        const float y = (mode == 2) ? ex[slot_of<NDIM, K>(p)] : os[wv][lane * OUT + slot_of<NDIM, K>(p)];
-        r = fminf(fmaxf(6.f * y, 0.f), 6.f);
-      }
-      RF[(long long)p * g.fs + ic] = r;
+  for (int nb = 0; nb < WAVE / 16; nb++) {
+    const int src = 16 * nb + r;
+    const long long ics = __shfl(ic, src, 64);
+    const int uses = __shfl((int)use, src, 64);
+    const int acts = __shfl((int)active, src, 64);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int p = var_of_slot<NDIM, K>(4 * q + i);
+      if (acts && p >= 0)
+        RF[(long long)p * g.fs + ics] = uses ? fminf(fmaxf(6.f * out[nb][i], 0.f), 6.f) : -1.f;
     }
   }
 }

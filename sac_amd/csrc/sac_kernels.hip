@@ -92,11 +92,11 @@ __global__ void __launch_bounds__(256) SAC_AXIS_OCC k_axis(AxisArgs a) {
 #endif
 
 // kernel 1, 2-D and 3-D grids: LDS-staged line tiles of axis K (see line_tile)
-template <int NDIM, int NDIR, int K, typename S, typename C>
+template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 __global__ void __launch_bounds__(line::NT) SAC_LINE_OCC k_line(AxisArgs a, int seg) {
   __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::RR * line::W];
   C numax = 0;
-  line::line_tile<NDIM, NDIR, K, S, C>(a, seg, L, numax);
+  line::line_tile<NDIM, NDIR, K, S, C, LEARNED>(a, seg, L, numax);
   if (a.visc_on) {
     double r2[1] = {(double)numax};
     block_max_atomic<1>(r2, a.red_visc);
@@ -114,14 +114,14 @@ static int line_segment(const GridDesc& g) {
   return (ext + nseg - 1) / nseg;
 }
 
-template <int NDIM, int NDIR, int K, typename S, typename C>
+template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 static void launch_line(const AxisArgs& A, hipStream_t st) {
   const int seg = line_segment<NDIM, K>(A.g);
   long long nb = line::nblocks<NDIM, K>(A.g, seg);
   nb = (nb + 15) / 16 * 16;  // tile pairing over the XCDs (see line_tile)
   if (nb > 0)
-    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C>), dim3((unsigned)nb), dim3(line::NT), 0, st, A,
-                       seg);
+    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C, LEARNED>), dim3((unsigned)nb), dim3(line::NT),
+                       0, st, A, seg);
 }
 
 // kernel 2: the combining update of every mesh cell (see final_cell)
@@ -285,14 +285,21 @@ static void launch_nd(int which, int axis, const void* args, hipStream_t st, lon
     const AxisArgs& A = *(const AxisArgs*)args;
     if (A.RF) {  // learned closure (FP64 storage only)
       if constexpr (sizeof(S) == sizeof(double)) {
-        if (axis == 0)
-          hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
-        if constexpr (NDIM > 1)
-          if (axis == 1)
-            hipLaunchKernelGGL((k_axis<NDIM, NDIR, 1, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
-        if constexpr (NDIM > 2)
-          if (axis == 2)
-            hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
+        if constexpr (NDIM > 1 && SAC_LINE_TILES) {
+          if (axis == 0) launch_line<NDIM, NDIR, 0, S, C, true>(A, st);
+          if (axis == 1) launch_line<NDIM, NDIR, 1, S, C, true>(A, st);
+          if constexpr (NDIM > 2)
+            if (axis == 2) launch_line<NDIM, NDIR, 2, S, C, true>(A, st);
+        } else {
+          if (axis == 0)
+            hipLaunchKernelGGL((k_axis<NDIM, NDIR, 0, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
+          if constexpr (NDIM > 1)
+            if (axis == 1)
+              hipLaunchKernelGGL((k_axis<NDIM, NDIR, 1, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
+          if constexpr (NDIM > 2)
+            if (axis == 2)
+              hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
+        }
       }
       return;
     }
@@ -356,7 +363,7 @@ int launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
     n *= e;
   }
   dispatch(a.prec, g.ndim, g.ndir, 0, axis, &a, st, n);
-  const bool line = SAC_LINE_TILES && g.ndim > 1 && !a.RF;
+  const bool line = SAC_LINE_TILES && g.ndim > 1;
   return (line && a.cross_last && a.visc_on && axis == g.ndim - 1) ? 1 : 0;
 }
 

@@ -88,7 +88,9 @@ __host__ __device__ inline long long nblocks(const GridDesc& g, int seg) {
   return nA * nB * nK;
 }
 
-template <int NDIM, int NDIR, int K, typename S, typename C>
+// LEARNED: the face ratios of setnu come from a.RF (learned closure, FP32,
+// stored at the face's left cell; < 0 where the exact ratio is required).
+template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restrict__ L,
                                           C& numax) {
   using SL = Slots<NDIM, NDIR>;
@@ -203,6 +205,17 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
           Tc[4 * k + 2] = (C)Tout[xi + fs + ipc];
           Tc[4 * k + 3] = (C)Tout[xi + fs + imc];
         }
+      }
+    }
+    // learned face ratios of the cell's two faces (c-1/2, c+1/2) per variable
+    float rfc[LEARNED ? 2 * NP : 1];
+    if constexpr (LEARNED) {
+      const int cc = cell_clamped(j);
+      const long long il = base + (long long)((cc - 1 >= 0) ? cc - 1 : 0) * sk;
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        rfc[2 * p] = a.RF[(long long)p * fs + il];
+        rfc[2 * p + 1] = a.RF[(long long)p * fs + icell];
       }
     }
     if (j + 1 < nch) load_pt<NDIR, S, C>(Wf, Bg, fs, point_index(j + 1), nxt);
@@ -322,8 +335,16 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
           for (int s = 0; s < 7; s++) q[s] = L[at<K>(SL::Q0 + p, rw[s], iw)];
           const C ch = (p >= 2 && p < 2 + NDIM) ? (C)0.4 : (C)0.04;
           const C Kc = ctk * ch;
-          const C nuL = Kc * face_ratio<C>(q) * dxk;
-          const C nuR = Kc * face_ratio<C>(q + 1) * dxk;
+          C ratL, ratR;
+          if constexpr (LEARNED) {
+            ratL = (rfc[2 * p] >= 0.f) ? (C)rfc[2 * p] : face_ratio<C>(q);
+            ratR = (rfc[2 * p + 1] >= 0.f) ? (C)rfc[2 * p + 1] : face_ratio<C>(q + 1);
+          } else {
+            ratL = face_ratio<C>(q);
+            ratR = face_ratio<C>(q + 1);
+          }
+          const C nuL = Kc * ratL * dxk;
+          const C nuR = Kc * ratR * dxk;
           numax = valid ? fmax(numax, fmax(nuR, nuL)) : numax;
           const C qm = q[2], q0 = q[3], qp = q[4];
           const C gL = z1 ? (C)0 : (q0 - qm) * idx;

@@ -117,6 +117,77 @@ __device__ __forceinline__ void forward(const uint16_t* P, const float* Pf, cons
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------
+// Inference with the activations in registers.  The D tile (mb, nb) of a
+// layer -- lane l holding Z[16mb + 4(l/16) + i][16nb + l%16] -- is laid out
+// exactly as the B operand tile (kb = mb, nb) of the next layer needs it, so
+// hidden activations go from the accumulators (bias, ReLU, BF16) straight
+// into the next layer's operands without LDS.  Only the input (per-lane
+// samples to K-major operand) passes through LDS.
+
+// one hidden layer: H[mb][nb] (BF16x4) from the previous layer's X[kb][nb]
+template <int M, int K>
+__device__ __forceinline__ void reg_layer(const uint16_t* __restrict__ W, const float* __restrict__ b,
+                                          const bf16x4 (&X)[K / 16][WAVE / 16],
+                                          bf16x4 (&H)[M / 16][WAVE / 16], int lane) {
+  const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int mb = 0; mb < M / 16; mb++) {
+    f32x4 acc[WAVE / 16];
+#pragma unroll
+    for (int nb = 0; nb < WAVE / 16; nb++) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < K / 16; kb++) {
+      const bf16x4 a = ld4(W + (16 * mb + r) * K + 16 * kb + 4 * q);
+#pragma unroll
+      for (int nb = 0; nb < WAVE / 16; nb++)
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, X[kb][nb], acc[nb], 0, 0, 0);
+    }
+    float bias[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) bias[i] = b[16 * mb + 4 * q + i];
+#pragma unroll
+    for (int nb = 0; nb < WAVE / 16; nb++) {
+      bf16x4 h;
+#pragma unroll
+      for (int i = 0; i < 4; i++) h[i] = (short)f2bf(fmaxf(acc[nb][i] + bias[i], 0.f));
+      H[mb][nb] = h;
+    }
+  }
+}
+
+// Forward pass of the wave's 64 samples from x (LDS, [64][IN] BF16).
+// Returns, in lane l, the outputs 4(l/16)+i (i = 0..3) of samples
+// 16nb + l%16 (nb = 0..3): out[nb][i].
+__device__ __forceinline__ void forward_regs(const uint16_t* __restrict__ P,
+                                             const float* __restrict__ Pf, const uint16_t* x,
+                                             f32x4 (&out)[WAVE / 16], int lane) {
+  const int r = lane & 15, q = lane >> 4;
+  bf16x4 X0[IN / 16][WAVE / 16];
+#pragma unroll
+  for (int kb = 0; kb < IN / 16; kb++)
+#pragma unroll
+    for (int nb = 0; nb < WAVE / 16; nb++) X0[kb][nb] = ld4(x + (16 * nb + r) * IN + 16 * kb + 4 * q);
+  bf16x4 H1[H / 16][WAVE / 16], H2[H / 16][WAVE / 16];
+  reg_layer<H, IN>(P + OFF_W1, Pf + OFF_B1, X0, H1, lane);
+  reg_layer<H, H>(P + OFF_W2, Pf + OFF_B2, H1, H2, lane);
+  reg_layer<H, H>(P + OFF_W3, Pf + OFF_B3, H2, H1, lane);
+  // linear output layer (M = OUT = 16: one row tile), FP32
+#pragma unroll
+  for (int nb = 0; nb < WAVE / 16; nb++) out[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < H / 16; kb++) {
+    const bf16x4 a = ld4(P + OFF_W4 + r * H + 16 * kb + 4 * q);
+#pragma unroll
+    for (int nb = 0; nb < WAVE / 16; nb++)
+      out[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, H1[kb][nb], out[nb], 0, 0, 0);
+  }
+#pragma unroll
+  for (int nb = 0; nb < WAVE / 16; nb++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[nb][i] += Pf[OFF_B4 + 4 * q + i];
+}
+
 // dX[s][k] = (sum_m W[m][k] dZ[s][m]) * relu'(X[s][k]), for the hidden layers.
 // W: BF16 [M][K]; dZ: LDS [64][M]; X: LDS [64][K] (the layer input, whose
 // positivity gives relu'); dX: LDS [64][K].
