@@ -69,6 +69,12 @@ module sac_bind
       real(c_double), value :: dt
     end function sac_dev_advance
 
+    integer(c_int) function sac_dev_set_divbfix(handle, on) bind(C, name='sac_dev_set_divbfix')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), value :: on
+    end function sac_dev_set_divbfix
+
     type(c_ptr) function sac_dev_last_error() bind(C, name='sac_dev_last_error')
       import :: c_ptr
     end function sac_dev_last_error

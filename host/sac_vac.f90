@@ -11,8 +11,8 @@
 !
 ! Configurations outside the GPU path stop with a message, as the
 ! reference's die() does: other schemes or splittings, resistivity,
-! Powell's div B source, smallfix, generalised coordinates, axial symmetry
-! and boundary types other than cont/fixed/periodic/symm/symm0/asymm.
+! smallfix, generalised coordinates, axial symmetry and the boundary types
+! zero and special.
 !=============================================================================
 module host_state
   use iso_c_binding
@@ -465,6 +465,8 @@ program sac_vac
        int(nu_, c_int), dxaxis, bcodes, merge(1_c_int, 0_c_int, fullgridini), adv, &
        SAC_PREC_FP64, h)
   call check(rc)
+  ! Powell's div B sources in addsource (src/vacphys.mhd0.t:256, ndim > 1)
+  call check(sac_dev_set_divbfix(h, merge(1_c_int, 0_c_int, divbfix)))
   call check(sac_dev_upload(h, w))
 
   ! startup (src/vac.t:161-221)
@@ -660,7 +662,6 @@ contains
     end select
     if (dimsplit) call die('dimsplit=T is not built: set dimsplit=F')
     if (sourcesplit .or. .not. sourceunsplit) call die('only sourceunsplit=T is built')
-    if (divbfix .and. ndim > 1) call die('divbfix=T (Powell source) is not built: set divbfix=F')
     if (smallfix) call die('smallfix=T is not built')
     if (typeaxial /= 'slab') call die('typeaxial=' // trim(typeaxial) // ' is not built')
     if (extraB) call die('extraB=T (special boundaries) is not built')

@@ -137,6 +137,11 @@ int sac_dev_getdt_special(sac_handle h, double *dt);
  * one full Runge-Kutta step of the state including every getboundary. */
 int sac_dev_advance(sac_handle h, double dt);
 
+/* divbfix: Powell's div B source terms (addsource_divb,
+ * src/vacphys.mhd0.t:256-297) added to every advect1 when on and ndim > 1
+ * (the reference adds nothing in 1-D).  Off after sac_dev_init. */
+int sac_dev_set_divbfix(sac_handle h, int on);
+
 /* advect1 with method 'cd4' (src/vac.t:490-549) on device registers:
  * slot w <- w + qdt*R(slot wCT), then getboundary(w). */
 int sac_dev_advect1(sac_handle h, double qdt, int wct_slot, int w_slot);

@@ -63,6 +63,7 @@ def lib():
             "oracle_set_w": (I, [P, dp]),
             "oracle_get_w": (I, [P, dp]),
             "oracle_set_iws": (I, [P, ip, I]),
+            "oracle_set_divbfix": (None, [P, I]),
             "oracle_getboundary": (None, [P]),
             "oracle_getboundary_on": (None, [P, dp]),
             "oracle_getdt_courant": (D, [P, D, dp]),
@@ -178,6 +179,9 @@ class Oracle:
     @maxviscoef.setter
     def maxviscoef(self, v):
         lib().oracle_set_maxviscoef(self._h, v)
+
+    def set_divbfix(self, on=True):
+        lib().oracle_set_divbfix(self._h, 1 if on else 0)
 
     def advance(self, dt, typeadvance="fourstep"):
         r = lib().oracle_advance(self._h, dt, ADV[typeadvance])

@@ -92,7 +92,9 @@ struct sac_oracle {
   double *sn_d1R, *sn_d1L, *sn_d3R, *sn_d3L;
   double *sn_md3R, *sn_md3L, *sn_md1R, *sn_md1L;
   double *dv_tmpdt, *dv_nushk;
+  double *db_divb;
   double *w1, *w2, *w3;
+  int divbfix;
 };
 
 /* ------------------------------------------------------------------ */
@@ -273,6 +275,7 @@ sac_oracle *oracle_new(int ndim, int ndir, const int *nx, const double *eqpar,
   o->sn_md1L = zalloc(n);
   o->dv_tmpdt = zalloc(n);
   o->dv_nushk = zalloc(3 * n);
+  o->db_divb = zalloc(n);
   o->w1 = zalloc(o->nw * n);
   o->w2 = zalloc(o->nw * n);
   o->w3 = zalloc(o->nw * n);
@@ -286,8 +289,8 @@ void oracle_free(sac_oracle *o) {
                   o->av_rhoR, o->av_rhoC, o->av_VL, o->av_VR, o->av_BL, o->av_BR, o->av_L,
                   o->av_R, o->av_C, o->av_nuL, o->av_nuR, o->sn_cmax, o->sn_tmp_nu,
                   o->sn_tmp_nuI, o->sn_d1R, o->sn_d1L, o->sn_d3R, o->sn_d3L, o->sn_md3R,
-                  o->sn_md3L, o->sn_md1R, o->sn_md1L, o->dv_tmpdt, o->dv_nushk, o->w1,
-                  o->w2, o->w3};
+                  o->sn_md3L, o->sn_md1R, o->sn_md1L, o->dv_tmpdt, o->dv_nushk, o->db_divb,
+                  o->w1, o->w2, o->w3};
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
   free(o);
 }
@@ -1024,12 +1027,58 @@ static void specialsource(sac_oracle *o, double qdt, box ixI, box ixO, const dou
   if (fabs(eq_nu(o)) > SMALLDOUBLE) addsource_visc(o, qdt, ixI, ixO, wCT, w);
 }
 
-/* addsource2 (src/vac.t:552-582); addsource (src/vacphys.mhd0.t:225) adds
- * nothing with eta=0 and divbfix=F */
+/* getdivb (src/vacphys.mhd0.t:106-146) with fourthorder=F: tmp = b+bg on
+ * ixO+1 and gradient4 along each axis (which zeroes its two outermost
+ * cells, the only ones that would read tmp beyond ixO+1) */
+static void getdivb(sac_oracle *o, const double *w, box ixO, double *divb) {
+  box ix = box_add(o, ixO, 1);
+  FOR_BOX(ixO) { C(divb) = 0.0; }
+  for (int idim = 1; idim <= o->ndim; idim++) {
+    FOR_BOX(ix) { C(o->tmp) = F(w, o->b0_ + idim) + F(w, o->bg0_ + idim); }
+    gradient4(o, o->tmp, ixO, idim, o->tmp2);
+    FOR_BOX(ixO) { C(divb) = C(divb) + C(o->tmp2); }
+  }
+}
+
+/* addsource_divb (src/vacphys.mhd0.t:264-297): Powell's div B sources,
+ * m -= (b+bg) qdt divB, b -= m/rho_t qdt divB, e -= (m.(b+bg))/rho_t qdt divB */
+static void addsource_divb(sac_oracle *o, double qdt, box ixO, const double *w, double *wnew) {
+  double *divb = o->db_divb;
+  getdivb(o, w, ixO, divb);
+  FOR_BOX(ixO) { C(divb) = qdt * C(divb); }
+  for (int ii = 0; ii < o->niws; ii++) {
+    int iw = o->iws[ii];
+    if (iw > o->m0_ && iw <= o->m0_ + o->ndir) {
+      int c = iw - o->m0_;
+      FOR_BOX(ixO) { F(wnew, iw) = F(wnew, iw) - (F(w, o->b0_ + c) + F(w, o->bg0_ + c)) * C(divb); }
+    } else if (iw > o->b0_ && iw <= o->b0_ + o->ndir) {
+      int c = iw - o->b0_;
+      FOR_BOX(ixO) {
+        F(wnew, iw) = F(wnew, iw) - F(w, o->m0_ + c) / (F(w, o->rho_) + F(w, o->rhob_)) * C(divb);
+      }
+    } else if (iw == o->e_) {
+      FOR_BOX(ixO) {
+        double s = 0.0;
+        for (int c = 1; c <= o->ndir; c++) {
+          double t = F(w, o->m0_ + c) * (F(w, o->b0_ + c) + F(w, o->bg0_ + c));
+          s = (c == 1) ? t : s + t;
+        }
+        F(wnew, iw) = F(wnew, iw) - s / (F(w, o->rho_) + F(w, o->rhob_)) * C(divb);
+      }
+    }
+  }
+}
+
+/* addsource2 (src/vac.t:552-582): specialsource, then addsource
+ * (src/vacphys.mhd0.t:225-260), which with eta=0 adds Powell's sources when
+ * divbfix and ndim > 1 */
 static void addsource2(sac_oracle *o, double qdt, box ixI, box ixO, const double *wCT,
                        double *w) {
   specialsource(o, qdt, ixI, ixO, wCT, w);
+  if (o->divbfix && o->ndim > 1) addsource_divb(o, qdt, ixO, wCT, w);
 }
+
+void oracle_set_divbfix(sac_oracle *o, int on) { o->divbfix = on ? 1 : 0; }
 
 /* centdiff4 (src/vaccd.t:7-78) with idim^LIM = 1..ndim */
 static void centdiff4(sac_oracle *o, double qdt, box ixI, box ixO, const double *wCT,

@@ -77,6 +77,8 @@ int oracle_get_w(const sac_oracle *o, double *w);
 
 /* iws list (1-based variable numbers); default all evolving variables */
 int oracle_set_iws(sac_oracle *o, const int *iws, int niws);
+/* Powell's div B source terms in addsource (divbfix, ndim > 1) */
+void oracle_set_divbfix(sac_oracle *o, int on);
 
 /* getboundary(t,1,nw,1,ndim,w) on the state (src/vacgrid.t:195) */
 void oracle_getboundary(sac_oracle *o);

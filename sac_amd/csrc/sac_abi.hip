@@ -79,6 +79,7 @@ struct Dev {
   double grav[3] = {0, 0, 0};
   int grav_on[3] = {0, 0, 0};
   int visc_on = 0;
+  int divbfix = 0;  // Powell's div B sources (addsource_divb), ndim > 1
   double dtgrav = kBigDouble;
   double dxmin[3] = {0, 0, 0};
   bool any_fixed = false;
@@ -224,6 +225,7 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
       x.RF = d.cl->RF[k];
     }
     x.cross_last = (k == d.ndim - 1) ? 1 : 0;
+    x.divbfix = d.divbfix;
     hipEvent_t e0{};
     t_begin(d, SAC_K_AXIS, &e0);
     skip_last = sac::launch_axis(x, k, d.stream);
@@ -714,6 +716,16 @@ int sac_dev_advance(sac_handle h, double dt) {
     }
   }
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int sac_dev_set_divbfix(sac_handle h, int on) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (on && d.ndim > 1 && !sac::line_tiles_built())
+    return fail(SAC_E_UNSUPPORTED, "divbfix needs the line-tile kernels");
+  // the reference adds nothing in 1-D ({^NOONED ...}, src/vacphys.mhd0.t:256)
+  d.divbfix = (on && d.ndim > 1) ? 1 : 0;
   return 0;
 }
 

@@ -22,6 +22,7 @@ struct AxisArgs {
   int xcd;                             // XCD count for block renumbering (1 = off)
   const float* RF;                     // learned face ratios of axis K, or null (exact)
   int cross_last;                      // last axis: add the cross-term derivatives along it
+  int divbfix;                         // Powell's div B sources (ndim > 1)
 };
 
 struct FinalArgs {
@@ -70,6 +71,8 @@ struct GhostArgs {
 // returns 1 if the cross-term derivatives along `axis` were added (requested
 // by a.cross_last and done by the line-tile kernel), else 0
 int launch_axis(const AxisArgs& a, int axis, hipStream_t st);
+// 1 if the line-tile kernels (which carry the Powell sources) are built
+int line_tiles_built();
 void launch_final(const FinalArgs& a, hipStream_t st);
 void launch_cmax(const CmaxArgs& a, hipStream_t st);
 void launch_ghost(const GhostArgs& a, hipStream_t st);

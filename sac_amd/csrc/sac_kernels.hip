@@ -351,6 +351,8 @@ static long long mesh_cells(const GridDesc& g) {
   return n;
 }
 
+int line_tiles_built() { return SAC_LINE_TILES ? 1 : 0; }
+
 int launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
   const GridDesc& g = a.g;
   long long n = 1;

@@ -53,7 +53,8 @@ struct Slots {
   static constexpr int RT = B0 + NDIM;  // rho_t
   static constexpr int M0 = RT + 1;     // m_l, l < NDIM
   static constexpr int Q0 = M0 + NDIM;  // setnu q of the NP variables
-  static constexpr int NV = Q0 + NP;
+  static constexpr int BT = Q0 + NP;    // b_K + bg_K (div B of Powell's sources)
+  static constexpr int NV = BT + 1;
 };
 
 template <int K>
@@ -234,6 +235,7 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
       L[at<K>(SL::PB, r0, iw)] = (C)ptotal_bg<NDIR, C>(pt, gamma);
       const double rtd = (double)pt.rho + pt.rhob;
       L[at<K>(SL::RT, r0, iw)] = (C)rtd;
+      L[at<K>(SL::BT, r0, iw)] = (C)((double)pt.b[K] + pt.bg[K]);
       {
 #pragma clang fp contract(off)
         // q of setnu: rho', thermal energy (in FP64), v_l, b_l (l != K)
@@ -293,6 +295,27 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
         R[1 + NDIR] += re;
 #pragma unroll
         for (int n = 0; n < NE; n++) R[n] = fluxc ? R[n] : (C)0;
+        // Powell's div B sources (addsource_divb, src/vacphys.mhd0.t:264-297):
+        // div B = sum over the axes of gradient4(b_K + bg_K) (zero on the two
+        // outermost cells like the flux), so each axis adds its share
+        //   m -= (b+bg) g_K,  b -= m/rho_t g_K,  e -= m.(b+bg)/rho_t g_K
+        // with the state of the cell (loaded here: all ndir components)
+        if (a.divbfix && valid && fluxc) {
+          const C D1b = L[at<K>(SL::BT, rp1, iw)] - L[at<K>(SL::BT, rm1, iw)];
+          const C D2b = L[at<K>(SL::BT, rp2, iw)] - L[at<K>(SL::BT, rm2, iw)];
+          const C gK = -((D2b - (C)8 * D1b) * inv12);
+          const double rt = (double)Wf[icell] + Bg[fs + icell];
+          double sm = 0.0;
+#pragma unroll
+          for (int cd = 0; cd < NDIR; cd++) {
+            const double m = (double)Wf[(1 + cd) * fs + icell];
+            const double bt = (double)Wf[(2 + NDIR + cd) * fs + icell] + Bg[(2 + cd) * fs + icell];
+            R[1 + cd] -= (C)bt * gK;
+            R[2 + NDIR + cd] -= (C)(m / rt) * gK;
+            sm = (cd == 0) ? m * bt : sm + m * bt;
+          }
+          R[1 + NDIR] -= (C)(sm / rt) * gK;
+        }
       }
       // The last axis runs after the others have written their cross-term
       // factors T_{k,K}: their derivatives along K (central, zero at a

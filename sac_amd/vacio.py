@@ -172,10 +172,11 @@ def read_log(path):
 
 def write_par(path, filenameini, fileout, filelog, *, typefileout="auto", itmax=None, tmax=None,
               tmaxexact=False, ditsave=None, dtsave=None, courantpar=0.3, typeadvance="fourstep",
-              typeB=None, nw=13, ne=8, ndim=3, wnames=None, dtpar=None, save_initial=True):
+              typeB=None, nw=13, ne=8, ndim=3, wnames=None, dtpar=None, save_initial=True,
+              divbfix=False, typepred=None):
     """A par file for the host program: the namelists of src/vacio.t:25-53
     with the method settings of the GPU path (cd4 on the evolving variables,
-    dimsplit=F, divbfix=F)."""
+    dimsplit=F; Powell's divbfix off unless asked)."""
     tB = typeB or ["cont"] * (2 * ndim)
     lines = ["&testlist /", "&filelist",
              f"  filenameini='{filenameini}'",
@@ -196,8 +197,11 @@ def write_par(path, filenameini, fileout, filelog, *, typefileout="auto", itmax=
     lines += ["/", "&methodlist",
               f"  wnames='{wnames or 'rho m1 m2 m3 e b1 b2 b3 eb rhob bg1 bg2 bg3'}'",
               f"  typeadvance='{typeadvance}'",
-              f"  typefull={ne}*'cd4',{nw - ne}*'nul'",
-              "  dimsplit=F", "  sourcesplit=F", "  divbfix=F", "/", "&boundlist"]
+              f"  typefull={ne}*'cd4',{nw - ne}*'nul'"]
+    if typepred is not None:
+        lines.append(f"  typepred={ne}*'{typepred}',{nw - ne}*'nul'")
+    lines += ["  dimsplit=F", "  sourcesplit=F", f"  divbfix={'T' if divbfix else 'F'}",
+              "  smallfix=F", "/", "&boundlist"]
     tbs = ",".join(f"{nw}*'{t}'" for t in tB)
     lines += [f"  typeB={tbs}", "/", "&paramlist", f"  courantpar={courantpar!r}"]
     if dtpar is not None:

@@ -79,13 +79,13 @@ def test_host_parses_then_needs_device(exe, tmp_path):
 
 @pytest.mark.parametrize("edit,msg", [
     ("dimsplit=F", "dimsplit=T is not built"),
-    ("divbfix=F", "divbfix=T (Powell source) is not built"),
+    ("smallfix=F", "smallfix=T is not built"),
     ("typeadvance='fourstep'", "typeadvance=adams2 is not built"),
 ])
 def test_host_refuses_unbuilt_methods(exe, tmp_path, edit, msg):
     prob = problems.stratified(n=6, nxyz=(6, 6, 6))
     par = _setup(tmp_path, prob, itmax=1)
-    repl = {"dimsplit=F": "dimsplit=T", "divbfix=F": "divbfix=T",
+    repl = {"dimsplit=F": "dimsplit=T", "smallfix=F": "smallfix=T",
             "typeadvance='fourstep'": "typeadvance='adams2'"}[edit]
     r = _run(exe, tmp_path, par.replace(edit, repl))
     assert r.returncode != 0

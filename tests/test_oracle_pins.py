@@ -825,3 +825,50 @@ def test_extrapolating_and_mirror_boundaries(oracle_mod):
     np.testing.assert_allclose(r[2, -2:], [q2[-1] + (hi[0] - q[-1]), q2[-1] + (hi[1] - q[-1])],
                                rtol=1e-15)
     np.testing.assert_array_equal(r[3, :2], [q2[2], q2[1]])
+
+
+def test_powell_divb_source_closed_form(oracle_mod):
+    """addsource_divb (src/vacphys.mhd0.t:264-297): with b+bg linear in
+    x and y, div B is exact for gradient4 away from the two zeroed edge
+    cells per side, and one advect1 with divbfix differs from the same step
+    without it by -qdt*divB*((b+bg) for m, m/rho_t for b, m.(b+bg)/rho_t for
+    e), nothing else"""
+    prob, v = _blank(2, 3, (12, 10), bc="cont")
+    x, y = _coords(prob, 0), _coords(prob, 1)
+    w = prob.w
+    w[0] = 0.3 + 0.1 * x            # rho'
+    w[v["rhob"] - 1] = 1.2
+    w[v["m0"]] = 0.2 - 0.1 * y
+    w[v["m0"] + 1] = 0.1 + 0.05 * x
+    w[v["m0"] + 2] = -0.3
+    w[v["e"] - 1] = 2.0
+    s1, t1, s2 = 0.7, -0.4, 0.25
+    w[v["b0"]] = 0.3 + s1 * x
+    w[v["b0"] + 1] = -0.2 + s2 * y
+    w[v["b0"] + 2] = 0.15 * x
+    w[v["bg0"]] = 0.2 + t1 * x
+    w[v["bg0"] + 1] = 0.05
+    w[v["bg0"] + 2] = 0.1
+    w[v["eb"] - 1] = 3.0
+    o = make_oracle(prob, oracle_mod)
+    wct = o.get_w()
+    qdt = 1e-3
+    off = o.advect1(qdt, wct, wct)
+    o.set_divbfix(True)
+    on = o.advect1(qdt, wct, wct)
+    o.set_divbfix(False)
+    d = on - off
+    divb = s1 + t1 + s2
+    rt = wct[0] + wct[v["rhob"] - 1]
+    exp = np.zeros_like(d)
+    for c in range(3):
+        bt = wct[v["b0"] + c] + wct[v["bg0"] + c]
+        exp[v["m0"] + c] = -qdt * bt * divb
+        exp[v["b0"] + c] = -qdt * wct[v["m0"] + c] / rt * divb
+        exp[v["e"] - 1] += -qdt * wct[v["m0"] + c] * bt / rt * divb
+    inner = (0, slice(4, -4), slice(4, -4))
+    for iw in range(v["nevolve"]):
+        scale = np.abs(exp[iw][inner]).max() + 1e-300
+        np.testing.assert_allclose(d[iw][inner], exp[iw][inner], atol=1e-10 * scale, rtol=0,
+                                   err_msg=f"var {iw + 1}")
+    assert np.abs(d[0][inner]).max() == 0.0  # rho' has no Powell term
