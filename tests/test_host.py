@@ -148,3 +148,30 @@ def test_host_stratified_binary_in_ascii_out(exe, tmp_path, oracle_mod):
     for iw in range(8):
         scale = np.abs(refm[iw]).max()
         assert np.abs(snaps[-1].w[iw] - refm[iw]).max() <= 1e-10 * scale + 5e-16, iw
+
+
+@pytest.mark.gpu
+def test_host_threestep_divbfix_extrapolating_boundaries(exe, tmp_path, oracle_mod):
+    """The drop-in program with settings beyond the plain path: Shu's
+    threestep integrator, Powell's div B sources (divbfix=T, the
+    reference's default) and cont1/grad1 boundaries along the vertical axis,
+    free-running against the oracle with the same settings."""
+    prob = problems.stratified(n=12, nxyz=(12, 10, 8), amp=3e-2)
+    prob.typeB[:, 0] = problems.BC["cont1"]
+    prob.typeB[:, 1] = problems.BC["grad1"]
+    steps = 6
+    par = _setup(tmp_path, prob, fmt="binary", out_fmt="binary", itmax=steps,
+                 typeadvance="threestep", divbfix=True)
+    r = _run(exe, tmp_path, par)
+    assert r.returncode == 0, r.stdout + r.stderr
+    snaps = vacio.read_binary(tmp_path / "out.binary")
+    assert snaps[-1].it == steps
+    o = make_oracle(prob, oracle_mod)
+    o.set_divbfix(True)
+    dts = [o.step(prob.courantpar, "threestep") for _ in range(steps)]
+    assert snaps[-1].t == pytest.approx(sum(dts), rel=1e-12)
+    ref = o.get_w()
+    got = prob.w.copy()
+    got[(Ellipsis,) + prob.mesh_slices()] = snaps[-1].w
+    errs = per_var_errors(prob, got, ref, 8)
+    assert max(errs) <= 1e-12, errs
