@@ -64,6 +64,8 @@ def lib():
             "oracle_get_w": (I, [P, dp]),
             "oracle_set_iws": (I, [P, ip, I]),
             "oracle_set_divbfix": (None, [P, I]),
+            "oracle_set_resistive": (None, [P, I, D]),
+            "oracle_getdt_res": (D, [P, D]),
             "oracle_getboundary": (None, [P]),
             "oracle_getboundary_on": (None, [P, dp]),
             "oracle_getdt_courant": (D, [P, D, dp]),
@@ -179,6 +181,12 @@ class Oracle:
     @maxviscoef.setter
     def maxviscoef(self, v):
         lib().oracle_set_maxviscoef(self._h, v)
+
+    def set_resistive(self, compactres=False, dtdiffpar=0.5):
+        lib().oracle_set_resistive(self._h, 1 if compactres else 0, dtdiffpar)
+
+    def getdt_res(self, dt):
+        return lib().oracle_getdt_res(self._h, dt)
 
     def set_divbfix(self, on=True):
         lib().oracle_set_divbfix(self._h, 1 if on else 0)

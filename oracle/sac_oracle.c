@@ -93,8 +93,11 @@ struct sac_oracle {
   double *sn_md3R, *sn_md3L, *sn_md1R, *sn_md1L;
   double *dv_tmpdt, *dv_nushk;
   double *db_divb;
+  double *rs_current; /* [3][n], components 1..3 (getcurrent's common array) */
   double *w1, *w2, *w3;
   int divbfix;
+  int compactres;     /* resistive sources: addsource_res1 (T) or res2 (F) */
+  double dtdiffpar;   /* common dtdiffpar: the par value until getdt_visc sets 0.25 */
 };
 
 /* ------------------------------------------------------------------ */
@@ -276,6 +279,8 @@ sac_oracle *oracle_new(int ndim, int ndir, const int *nx, const double *eqpar,
   o->dv_tmpdt = zalloc(n);
   o->dv_nushk = zalloc(3 * n);
   o->db_divb = zalloc(n);
+  o->rs_current = zalloc(3 * n);
+  o->dtdiffpar = 0.5; /* par/DEFAULT */
   o->w1 = zalloc(o->nw * n);
   o->w2 = zalloc(o->nw * n);
   o->w3 = zalloc(o->nw * n);
@@ -290,7 +295,7 @@ void oracle_free(sac_oracle *o) {
                   o->av_R, o->av_C, o->av_nuL, o->av_nuR, o->sn_cmax, o->sn_tmp_nu,
                   o->sn_tmp_nuI, o->sn_d1R, o->sn_d1L, o->sn_d3R, o->sn_d3L, o->sn_md3R,
                   o->sn_md3L, o->sn_md1R, o->sn_md1L, o->dv_tmpdt, o->dv_nushk, o->db_divb,
-                  o->w1, o->w2, o->w3};
+                  o->rs_current, o->w1, o->w2, o->w3};
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
   free(o);
 }
@@ -1069,13 +1074,165 @@ static void addsource_divb(sac_oracle *o, double qdt, box ixO, const double *w, 
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* resistive MHD with a constant eta = eqpar(eta_) > 0
+ * (src/vacphys.mhdres.t; eta < 0 needs the user's specialeta) */
+
+static double eq_eta(const sac_oracle *o) { return o->eqpar[1]; }
+
+/* Levi-Civita symbol lvc(i,j,k), 1-based */
+static int lvc(int i, int j, int k) {
+  if (i == j || j == k || i == k) return 0;
+  return ((i == 1 && j == 2) || (i == 2 && j == 3) || (i == 3 && j == 1)) ? 1 : -1;
+}
+
+/* the module's own gradient (src/vacphys.mhdres.t:304-317): the NEGATIVE
+ * central difference -(q(i+1)-q(i-1))/dx/2, no edge zeroing */
+static void gradient_res(sac_oracle *o, const double *q, box ix, int idim, double *gq) {
+  FOR_BOX(ix) { C(gq) = -(SH(q, idim, 1) - SH(q, idim, -1)) / DX(idim) / 2.0; }
+}
+
+/* getcurrent (src/vacphys.mhdres.t:48-93): J_i = sum lvc(i,j,k) gradient(b_k+bg_k)_j
+ * on ix, components idirmin0..3 (idirmin0 = 7-2 ndir); returns idirmin */
+static int getcurrent(sac_oracle *o, const double *w, box ix) {
+  int idirmin0 = 7 - 2 * o->ndir, idirmin = 4;
+  box ixI = box_add(o, ix, 1);
+  for (int idir = idirmin0; idir <= 3; idir++) {
+    double *J = o->rs_current + (long)(idir - 1) * o->n;
+    FOR_BOX(ix) { C(J) = 0.0; }
+  }
+  for (int idir = idirmin0; idir <= 3; idir++)
+    for (int jdir = 1; jdir <= o->ndim; jdir++)
+      for (int kdir = 1; kdir <= o->ndir; kdir++) {
+        int l = lvc(idir, jdir, kdir);
+        if (l == 0) continue;
+        double *J = o->rs_current + (long)(idir - 1) * o->n;
+        FOR_BOX(ixI) { C(o->tmp) = F(w, o->b0_ + kdir) + F(w, o->bg0_ + kdir); }
+        gradient_res(o, o->tmp, ix, jdir, o->tmp2);
+        if (l == 1) {
+          FOR_BOX(ix) { C(J) = C(J) + C(o->tmp2); }
+        } else {
+          FOR_BOX(ix) { C(J) = C(J) - C(o->tmp2); }
+        }
+        if (idir < idirmin) idirmin = idir;
+      }
+  return idirmin;
+}
+
+/* addsource_res2 (src/vacphys.mhdres.t:223-301), conservative 5-point form:
+ * b_i -= sum lvc(i,j,k) gradient(J_k eta qdt)_j,
+ * e += sum lvc(i,j,k) gradient((b_j+bg_j) J_k eta qdt)_i */
+static void addsource_res2(sac_oracle *o, double qdt, box ixO, const double *w, double *wnew) {
+  box ix = box_add(o, ixO, 1);
+  int idirmin = getcurrent(o, w, ix);
+  double eta = eq_eta(o);
+  for (int ii = 0; ii < o->niws; ii++) {
+    int iw = o->iws[ii];
+    if (iw > o->b0_ && iw <= o->b0_ + o->ndir) {
+      int idir = iw - o->b0_;
+      for (int jdir = 1; jdir <= o->ndim; jdir++)
+        for (int kdir = idirmin; kdir <= 3; kdir++) {
+          int l = lvc(idir, jdir, kdir);
+          if (l == 0) continue;
+          const double *J = o->rs_current + (long)(kdir - 1) * o->n;
+          FOR_BOX(ix) { C(o->tmp) = C(J) * eta * qdt; }
+          gradient_res(o, o->tmp, ixO, jdir, o->tmp2);
+          if (l == 1) {
+            FOR_BOX(ixO) { F(wnew, iw) = F(wnew, iw) - C(o->tmp2); }
+          } else {
+            FOR_BOX(ixO) { F(wnew, iw) = F(wnew, iw) + C(o->tmp2); }
+          }
+        }
+    } else if (iw == o->e_) {
+      for (int idir = 1; idir <= o->ndim; idir++)
+        for (int jdir = 1; jdir <= o->ndir; jdir++)
+          for (int kdir = idirmin; kdir <= 3; kdir++) {
+            int l = lvc(idir, jdir, kdir);
+            if (l == 0) continue;
+            const double *J = o->rs_current + (long)(kdir - 1) * o->n;
+            FOR_BOX(ix) {
+              C(o->tmp) = (F(w, o->b0_ + jdir) + F(w, o->bg0_ + jdir)) * C(J) * eta * qdt;
+            }
+            gradient_res(o, o->tmp, ixO, idir, o->tmp2);
+            if (l == 1) {
+              FOR_BOX(ixO) { F(wnew, iw) = F(wnew, iw) + C(o->tmp2); }
+            } else {
+              FOR_BOX(ixO) { F(wnew, iw) = F(wnew, iw) - C(o->tmp2); }
+            }
+          }
+    }
+  }
+}
+
+/* addsource_res1 (src/vacphys.mhdres.t:96-220), compact 3-point form:
+ * b_i += qdt eta lapl(b_i+bg_i), e += qdt eta lapl(b_i+bg_i) (b_i+bg_i),
+ * then e += qdt eta J^2 */
+static void addsource_res1(sac_oracle *o, double qdt, box ix, const double *w, double *wnew) {
+  int idirmin = getcurrent(o, w, ix);
+  double eta = eq_eta(o);
+  box g = grid_box(o);
+  for (int idir = 1; idir <= o->ndir; idir++) {
+    FOR_BOX(ix) { C(o->tmp) = 0.0; }
+    FOR_BOX(g) { C(o->tmp2) = F(w, o->b0_ + idir) + F(w, o->bg0_ + idir); }
+    for (int idim = 1; idim <= o->ndim; idim++) {
+      FOR_BOX(ix) {
+        C(o->tmp) = C(o->tmp) + (SH(o->tmp2, idim, 1) - 2 * C(o->tmp2) + SH(o->tmp2, idim, -1)) /
+                                    (DX(idim) * DX(idim));
+      }
+    }
+    FOR_BOX(ix) { C(o->tmp) = C(o->tmp) * eta; }
+    for (int ii = 0; ii < o->niws; ii++) {
+      int iw = o->iws[ii];
+      if (iw == o->b0_ + idir) {
+        FOR_BOX(ix) { F(wnew, iw) = F(wnew, iw) + qdt * C(o->tmp); }
+      } else if (iw == o->e_) {
+        FOR_BOX(ix) {
+          F(wnew, iw) = F(wnew, iw) + qdt * C(o->tmp) * (F(w, o->b0_ + idir) + F(w, o->bg0_ + idir));
+        }
+      }
+    }
+  }
+  for (int ii = 0; ii < o->niws; ii++) {
+    if (o->iws[ii] != o->e_) continue;
+    FOR_BOX(ix) { C(o->tmp) = 0.0; }
+    for (int idir = idirmin; idir <= 3; idir++) {
+      const double *J = o->rs_current + (long)(idir - 1) * o->n;
+      FOR_BOX(ix) { C(o->tmp) = C(o->tmp) + C(J) * C(J); }
+    }
+    FOR_BOX(ix) { F(wnew, o->e_) = F(wnew, o->e_) + qdt * eta * C(o->tmp); }
+  }
+}
+
+/* getdt_res (src/vacphys.mhdres.t:5-44) for eta > 0: dtdiffpar min(dx)^2/eta */
+static double getdt_res(sac_oracle *o, double dt) {
+  double eta = eq_eta(o);
+  if (eta == 0.0) return dt;
+  box ix = mesh_box(o);
+  double mn = BIGDOUBLE;
+  for (int idim = 1; idim <= o->ndim; idim++) FOR_BOX(ix) {
+      if (DX(idim) < mn) mn = DX(idim);
+    }
+  return fmin(dt, o->dtdiffpar * mn * mn / eta);
+}
+
 /* addsource2 (src/vac.t:552-582): specialsource, then addsource
- * (src/vacphys.mhd0.t:225-260), which with eta=0 adds Powell's sources when
- * divbfix and ndim > 1 */
+ * (src/vacphys.mhd0.t:225-260): the resistive sources when eta /= 0, then
+ * Powell's sources when divbfix and ndim > 1 */
 static void addsource2(sac_oracle *o, double qdt, box ixI, box ixO, const double *wCT,
                        double *w) {
   specialsource(o, qdt, ixI, ixO, wCT, w);
+  if (fabs(eq_eta(o)) > SMALLDOUBLE) {
+    if (o->compactres)
+      addsource_res1(o, qdt, ixO, wCT, w);
+    else
+      addsource_res2(o, qdt, ixO, wCT, w);
+  }
   if (o->divbfix && o->ndim > 1) addsource_divb(o, qdt, ixO, wCT, w);
+}
+
+void oracle_set_resistive(sac_oracle *o, int compactres, double dtdiffpar) {
+  o->compactres = compactres ? 1 : 0;
+  o->dtdiffpar = dtdiffpar;
 }
 
 void oracle_set_divbfix(sac_oracle *o, int on) { o->divbfix = on ? 1 : 0; }
@@ -1209,7 +1366,10 @@ static double getdt_visc(sac_oracle *o, double dt) {
     double *nk = o->dv_nushk + (long)(idim - 1) * o->n;
     FOR_BOX(ix) { C(nk) = 0.0; }
   }
-  double dtdiffpar = 0.25;
+  /* dtdiffpar=0.25d0 assigns the common variable, which the resistive dt
+   * limit of later steps then uses */
+  o->dtdiffpar = 0.25;
+  double dtdiffpar = o->dtdiffpar;
   for (int idim = 1; idim <= o->ndim; idim++) {
     double *nk = o->dv_nushk + (long)(idim - 1) * o->n;
     FOR_BOX(ix) { C(o->dv_tmpdt) = o->maxviscoef + C(nk); }
@@ -1246,6 +1406,8 @@ static double getdt_grav(sac_oracle *o, double dt) {
   return fmin(dt, o->dtgrav);
 }
 
+double oracle_getdt_res(sac_oracle *o, double dt) { return getdt_res(o, dt); }
+
 double oracle_getdt_special(sac_oracle *o, double dt) {
   if (fabs(eq_nu(o)) > SMALLDOUBLE) dt = getdt_visc(o, dt);
   dt = getdt_grav(o, dt);
@@ -1268,7 +1430,8 @@ int oracle_it(const sac_oracle *o) { return o->it; }
 double oracle_step(sac_oracle *o, double courantpar, int typeadvance) {
   double dt = BIGDOUBLE;
   if (courantpar > 0.0) dt = oracle_getdt_courant(o, courantpar, NULL);
-  /* getdt (src/vacphys.mhd0.t:83): eta=0 returns at once */
+  /* getdt (src/vacphys.mhd0.t:83-103): the resistive limit when eta /= 0 */
+  dt = getdt_res(o, dt);
   dt = oracle_getdt_special(o, dt);
   oracle_advance(o, dt, typeadvance);
   o->it += 1;

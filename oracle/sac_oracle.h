@@ -79,6 +79,10 @@ int oracle_get_w(const sac_oracle *o, double *w);
 int oracle_set_iws(sac_oracle *o, const int *iws, int niws);
 /* Powell's div B source terms in addsource (divbfix, ndim > 1) */
 void oracle_set_divbfix(sac_oracle *o, int on);
+/* resistive sources (eta = eqpar(2) > 0): compactres selects addsource_res1,
+ * else res2; dtdiffpar is the par value (getdt_visc later sets 0.25) */
+void oracle_set_resistive(sac_oracle *o, int compactres, double dtdiffpar);
+double oracle_getdt_res(sac_oracle *o, double dt);
 
 /* getboundary(t,1,nw,1,ndim,w) on the state (src/vacgrid.t:195) */
 void oracle_getboundary(sac_oracle *o);

@@ -872,3 +872,58 @@ def test_powell_divb_source_closed_form(oracle_mod):
         np.testing.assert_allclose(d[iw][inner], exp[iw][inner], atol=1e-10 * scale, rtol=0,
                                    err_msg=f"var {iw + 1}")
     assert np.abs(d[0][inner]).max() == 0.0  # rho' has no Powell term
+
+
+@pytest.mark.parametrize("compact", [False, True])
+def test_resistive_sources_closed_form(oracle_mod, compact):
+    """Resistive MHD with constant eta (src/vacphys.mhdres.t): for
+    B_y = c x^2 (centred differences of a quadratic are exact) one advect1
+    differs from the same step with eta=0 by b_y += qdt eta 2c and, for the
+    energy, by the discrete closed forms of the two source forms: res2
+    (conservative) e += qdt eta 2c^2 (3x^2 + h^2), res1 (compact) e += qdt
+    eta 6 c^2 x^2; nothing else changes"""
+    prob, v = _blank(2, 3, (16, 12), bc="cont")
+    x = _coords(prob, 0)
+    h = 1.0 / prob.nx[0]
+    c2, eta, qdt = 0.8, 2e-3, 1e-3
+    w = prob.w
+    w[0] = 0.1
+    w[v["rhob"] - 1] = 1.0
+    w[v["e"] - 1] = 2.0
+    w[v["eb"] - 1] = 1.0
+    w[v["b0"] + 1] = c2 * x * x
+    prob.eqpar[1] = eta
+    o = make_oracle(prob, oracle_mod)
+    o.set_resistive(compact, 0.5)
+    wct = o.get_w()
+    on = o.advect1(qdt, wct, wct)
+    prob0 = type(prob)(**{**prob.__dict__, "eqpar": prob.eqpar.copy()})
+    prob0.eqpar[1] = 0.0
+    off = make_oracle(prob0, oracle_mod).advect1(qdt, wct, wct)
+    d = on - off
+    inner = (0, slice(5, -5), slice(5, -5))
+    xi = x[inner]
+    np.testing.assert_allclose(d[v["b0"] + 1][inner], qdt * eta * 2 * c2, rtol=1e-9)
+    if compact:
+        exp_e = qdt * eta * 6 * c2 * c2 * xi * xi
+    else:
+        exp_e = qdt * eta * 2 * c2 * c2 * (3 * xi * xi + h * h)
+    np.testing.assert_allclose(d[v["e"] - 1][inner], exp_e, rtol=1e-9, atol=1e-22)
+    for iw in range(v["nevolve"]):
+        if iw not in (v["b0"] + 1, v["e"] - 1):
+            assert np.abs(d[iw][inner]).max() < 1e-20, iw
+
+
+def test_resistive_dt_limit(oracle_mod):
+    """getdt_res = dtdiffpar min(dx)^2 / eta with the par value of dtdiffpar
+    until getdt_visc has assigned 0.25 to it"""
+    prob, v = _blank(2, 3, (16, 12), bc="cont", nu=1.0)
+    prob.w[v["rhob"] - 1] = 1.0
+    prob.w[v["e"] - 1] = 2.0
+    prob.eqpar[1] = 2e-3
+    o = make_oracle(prob, oracle_mod)
+    o.set_resistive(False, 0.6)
+    dxmin = min(1.0 / 16, 1.0 / 12)
+    assert o.getdt_res(1e30) == pytest.approx(0.6 * dxmin ** 2 / 2e-3, rel=1e-14)
+    o.getdt_special(1e30)  # getdt_visc sets the common dtdiffpar to 0.25
+    assert o.getdt_res(1e30) == pytest.approx(0.25 * dxmin ** 2 / 2e-3, rel=1e-14)
