@@ -75,6 +75,14 @@ module sac_bind
       integer(c_int), value :: on
     end function sac_dev_set_divbfix
 
+    integer(c_int) function sac_dev_set_resistive(handle, compactres, dtdiffpar) &
+         bind(C, name='sac_dev_set_resistive')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int), value :: compactres
+      real(c_double), value :: dtdiffpar
+    end function sac_dev_set_resistive
+
     type(c_ptr) function sac_dev_last_error() bind(C, name='sac_dev_last_error')
       import :: c_ptr
     end function sac_dev_last_error

@@ -10,9 +10,9 @@
 ! the GPU through libsac_amd.so (include/sac_amd.h); the host owns w.
 !
 ! Configurations outside the GPU path stop with a message, as the
-! reference's die() does: other schemes or splittings, resistivity,
-! smallfix, generalised coordinates, axial symmetry and the boundary types
-! zero and special.
+! reference's die() does: other schemes or splittings, a resistivity given
+! by the user's specialeta (eqpar(eta_)<0), smallfix, generalised
+! coordinates, axial symmetry and the boundary types zero and special.
 !=============================================================================
 module host_state
   use iso_c_binding
@@ -467,6 +467,9 @@ program sac_vac
   call check(rc)
   ! Powell's div B sources in addsource (src/vacphys.mhd0.t:256, ndim > 1)
   call check(sac_dev_set_divbfix(h, merge(1_c_int, 0_c_int, divbfix)))
+  ! resistive sources (src/vacphys.mhd0.t:240-249) and the common dtdiffpar
+  call check(sac_dev_set_resistive(h, merge(1_c_int, 0_c_int, compactres), &
+       real(dtdiffpar, c_double)))
   call check(sac_dev_upload(h, w))
 
   ! startup (src/vac.t:161-221)
@@ -496,7 +499,8 @@ program sac_vac
         call check(sac_dev_getdt_courant(h, courantpar, dtc, cmax))
         dt = dtc
       end if
-      ! getdt: eta=0 returns at once; getdt_special: viscous and gravity limits
+      ! getdt (the resistive limit when eta > 0), then getdt_special (viscous
+      ! and gravity limits), in one call
       call check(sac_dev_getdt_special(h, dt))
       if (dtcantgrow .and. it > itmin) dt = min(dt, dtold)
       dtold = dt
@@ -665,7 +669,8 @@ contains
     if (smallfix) call die('smallfix=T is not built')
     if (typeaxial /= 'slab') call die('typeaxial=' // trim(typeaxial) // ' is not built')
     if (extraB) call die('extraB=T (special boundaries) is not built')
-    if (eqpar(2) /= 0.0_dp) call die('Resistive MHD (eqpar(eta_)/=0) is not built')
+    if (eqpar(2) < 0.0_dp) &
+         call die('Resistive MHD with eqpar(eta_)<0 (specialeta) is not built')
     if (nB /= 2 * ndim) call die('only the default boundary regions are built (nB=2*ndim)')
     if (residmin > 0.0_dp .or. residmax < bigdouble) &
          call die('steady-state residuals (residmin/residmax) are not built')

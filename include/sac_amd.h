@@ -142,6 +142,14 @@ int sac_dev_advance(sac_handle h, double dt);
  * (the reference adds nothing in 1-D).  Off after sac_dev_init. */
 int sac_dev_set_divbfix(sac_handle h, int on);
 
+/* Resistive MHD with a constant eta = eqpar(eta_) > 0 (set at sac_dev_init;
+ * eta < 0 needs the user's specialeta and is rejected there): compactres
+ * selects addsource_res1 (compact) over addsource_res2
+ * (src/vacphys.mhdres.t:96-301); dtdiffpar is the par value the diffusion
+ * limit dtdiffpar*min(dx)^2/eta uses until getdt_visc assigns 0.25 to it
+ * (getdt_res, src/vacphys.mhdres.t:5-44).  Defaults: res2, 0.5. */
+int sac_dev_set_resistive(sac_handle h, int compactres, double dtdiffpar);
+
 /* advect1 with method 'cd4' (src/vac.t:490-549) on device registers:
  * slot w <- w + qdt*R(slot wCT), then getboundary(w). */
 int sac_dev_advect1(sac_handle h, double qdt, int wct_slot, int w_slot);

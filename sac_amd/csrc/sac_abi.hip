@@ -80,6 +80,13 @@ struct Dev {
   int grav_on[3] = {0, 0, 0};
   int visc_on = 0;
   int divbfix = 0;  // Powell's div B sources (addsource_divb), ndim > 1
+  // resistive MHD (src/vacphys.mhdres.t): constant eta = eqpar(eta_) > 0,
+  // addsource_res1 (compactres) or res2, and the common dtdiffpar (the par
+  // value until getdt_visc assigns 0.25)
+  double eta = 0.0;
+  int compactres = 0;
+  double dtdiffpar = 0.5;
+  double* J = nullptr;  // current density scratch (3 fields)
   double dtgrav = kBigDouble;
   double dxmin[3] = {0, 0, 0};
   bool any_fixed = false;
@@ -231,6 +238,19 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
     skip_last = sac::launch_axis(x, k, d.stream);
     t_end(d, SAC_K_AXIS, e0, 5 + k);
   }
+  if (d.eta > 0.0) {  // resistive sources into R (addsource, src/vacphys.mhd0.t:240-249)
+    sac::ResistArgs ra{};
+    ra.g = d.g;
+    ra.prec = d.prec;
+    ra.w = d.reg[wct];
+    ra.bg = d.bg;
+    for (int k = 0; k < 3; k++) ra.dx[k] = d.dx[k];
+    ra.J = d.J;
+    ra.R = d.rate;
+    ra.eta = d.eta;
+    ra.compact = d.compactres;
+    sac::launch_resistive(ra, d.stream);
+  }
   sac::launch_zero_u64(red_of(d, out), sac::kRedSlots, d.stream);
   sac::FinalArgs s{};
   s.g = d.g;
@@ -355,7 +375,11 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
     if (iws[i] != i + 1) return fail(SAC_E_UNSUPPORTED, "iws must be 1..%d in order", ne);
   if (neqpar < 2) return fail(SAC_E_ARG, "neqpar=%d < 2", neqpar);
   if (eqpar[0] <= 0.0) return fail(SAC_E_ARG, "gamma must be > 0");
-  if (eqpar[1] != 0.0) return fail(SAC_E_UNSUPPORTED, "resistive MHD (eta /= 0) is not built");
+  if (eqpar[1] < 0.0)
+    return fail(SAC_E_UNSUPPORTED,
+                "resistive MHD with eqpar(eta_)<0 (the user's specialeta) is not built");
+  if (eqpar[1] > 0.0 && ndir < 2)
+    return fail(SAC_E_ARG, "resistive MHD needs ndir >= 2");
   if (typeadvance < SAC_ADV_ONESTEP || typeadvance > SAC_ADV_JAMESON)
     return fail(SAC_E_UNSUPPORTED, "typeadvance code %d not built", typeadvance);
   if (precision != SAC_PREC_FP64 && precision != SAC_PREC_MIXED)
@@ -430,6 +454,7 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   }
   double nu = (nu_ >= 1 && nu_ <= neqpar) ? eqpar[nu_ - 1] : 0.0;
   d->visc_on = std::fabs(nu) > kSmallDouble;
+  d->eta = eqpar[1];
   for (int iw = 0; iw < nw; iw++)
     for (int iB = 0; iB < 2 * ndim; iB++)
       if (typeB[iw * 2 * ndim + iB] == SAC_BC_FIXED || typeB[iw * 2 * ndim + iB] == SAC_BC_FIXED1 ||
@@ -446,6 +471,7 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
     if ((rc = ealloc(*d, &d->reg[s], (long long)ne * g.fs))) return bail(rc);
   if ((rc = dalloc(*d, &d->bg, (long long)d->nbg * g.fs))) return bail(rc);
   if ((rc = ealloc(*d, &d->rate, (long long)ne * g.fs))) return bail(rc);
+  if (d->eta > 0.0 && (rc = dalloc(*d, &d->J, 3LL * g.fs))) return bail(rc);
   if (d->visc_on && ndim > 1 &&
       (rc = ealloc(*d, &d->cross, (long long)2 * ndim * (ndim - 1) * g.fs)))
     return bail(rc);
@@ -521,6 +547,7 @@ int sac_dev_free(sac_handle h) {
   hipFree(d->bg);
   hipFree(d->rate);
   hipFree(d->cross);
+  hipFree(d->J);
   hipFree(d->snap);
   for (int k = 0; k < 3; k++) hipFree(d->dx[k]);
   hipFree(d->bc_e);
@@ -634,13 +661,22 @@ int sac_dev_getdt_special(sac_handle h, double* dt) {
   if (!dt) return fail(SAC_E_ARG, "dt is null");
   Dev& d = *(Dev*)h;
   double v = *dt;
+  // getdt (src/vacphys.mhd0.t:83-103) -> getdt_res (src/vacphys.mhdres.t:5-44),
+  // called before getdt_special in the main loop (src/vac.t:65-66)
+  if (d.eta > 0.0) {
+    double mn = kBigDouble;
+    for (int k = 0; k < d.ndim; k++) mn = std::fmin(mn, d.dxmin[k]);
+    v = std::fmin(v, d.dtdiffpar * mn * mn / d.eta);
+  }
   if (d.visc_on) {
     unsigned long long u = 0;
     HIPCHK(hipMemcpyAsync(&u, red_visc(d), sizeof(u), hipMemcpyDeviceToHost, d.stream));
     HIPCHK(hipStreamSynchronize(d.stream));
     double mv;
     memcpy(&mv, &u, sizeof(mv));
-    // getdt_visc: dtdiffpar forced to 0.25; +Inf when maxviscoef==0
+    // getdt_visc: dtdiffpar=0.25 (assigned to the common variable, which the
+    // resistive limit of later steps uses); +Inf when maxviscoef==0
+    d.dtdiffpar = 0.25;
     for (int k = 0; k < d.ndim; k++) {
       double mx = mv / (d.dxmin[k] * d.dxmin[k]);
       double dtd = 0.25 / mx;
@@ -716,6 +752,15 @@ int sac_dev_advance(sac_handle h, double dt) {
     }
   }
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int sac_dev_set_resistive(sac_handle h, int compactres, double dtdiffpar) {
+  if (int rc = check_handle(h)) return rc;
+  if (!(dtdiffpar > 0.0)) return fail(SAC_E_ARG, "dtdiffpar must be > 0");
+  Dev& d = *(Dev*)h;
+  d.compactres = compactres ? 1 : 0;
+  d.dtdiffpar = dtdiffpar;
   return 0;
 }
 

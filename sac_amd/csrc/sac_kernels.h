@@ -68,6 +68,20 @@ struct GhostArgs {
   int general;         // some fill has two sources (cont1; fixed1/grad1 before storage)
 };
 
+// resistive sources with a constant eta > 0 (sac_resist.hip)
+struct ResistArgs {
+  GridDesc g;
+  int prec;
+  const void* w;         // stage input
+  const double* bg;
+  const double* dx[3];
+  double* J;             // current density scratch, 3 FP64 fields
+  void* R;               // rate accumulator
+  double eta;
+  int compact;           // 1: addsource_res1, 0: addsource_res2
+};
+void launch_resistive(const ResistArgs& a, hipStream_t st);
+
 // returns 1 if the cross-term derivatives along `axis` were added (requested
 // by a.cross_last and done by the line-tile kernel), else 0
 int launch_axis(const AxisArgs& a, int axis, hipStream_t st);

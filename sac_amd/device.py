@@ -26,7 +26,7 @@ EXPORTS = [
     "sac_dev_init", "sac_dev_free", "sac_dev_upload", "sac_dev_download",
     "sac_dev_upload_slot", "sac_dev_download_slot", "sac_dev_getboundary",
     "sac_dev_getdt_courant", "sac_dev_getdt_special", "sac_dev_advance",
-    "sac_dev_advect1", "sac_dev_set_divbfix", "sac_dev_copy_slot", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
+    "sac_dev_advect1", "sac_dev_set_divbfix", "sac_dev_set_resistive", "sac_dev_copy_slot", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
     "sac_dev_timing", "sac_dev_timing_get", "sac_dev_synchronize",
     "sac_dev_memory", "sac_dev_last_error",
 ]
@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
         "sac_dev_advance": (I, [V, D]),
         "sac_dev_advect1": (I, [V, D, I, I]),
         "sac_dev_set_divbfix": (I, [V, I]),
+        "sac_dev_set_resistive": (I, [V, I, D]),
         "sac_dev_copy_slot": (I, [V, I, I]),
         "sac_dev_get_maxviscoef": (I, [V, dp]),
         "sac_dev_set_maxviscoef": (I, [V, D]),
@@ -199,6 +200,10 @@ class Device:
 
     def advect1(self, qdt, wct_slot, w_slot):
         _check(load_library().sac_dev_advect1(self._h, qdt, wct_slot, w_slot))
+
+    def set_resistive(self, compactres=False, dtdiffpar=0.5):
+        """Resistive source form and dtdiffpar (eta = eqpar[1] > 0)."""
+        _check(load_library().sac_dev_set_resistive(self._h, 1 if compactres else 0, dtdiffpar))
 
     def set_divbfix(self, on=True):
         """Powell's div B sources in every advect1 (ndim > 1)."""

@@ -48,7 +48,7 @@ def test_init_rejects_unsupported_configurations():
     h = ctypes.c_void_p()
     nx = np.array([20], dtype=np.int32)
     iws = np.arange(1, 7, dtype=np.int32)
-    eq = np.array([2.0, 0.5, 0, 0, 0, 1.0])  # eta /= 0 -> resistive MHD not built
+    eq = np.array([2.0, -0.5, 0, 0, 0, 1.0])  # eta < 0 (specialeta) -> not built
     dx = np.full(24, 0.05)
     tb = np.ones(20, dtype=np.int32)
     rc = L.sac_dev_init(1, 2, device._ip(nx), 10, device._ip(iws), 6, device._dp(eq), 6, 3, 6,

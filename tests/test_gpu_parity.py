@@ -259,3 +259,65 @@ def test_powell_divb_stage(dev_mod, oracle_mod, case):
     o.set_divbfix(False)
     ref_off = o.advect1(1e-3, wct, w0)
     assert max(per_var_errors(prob, ref_off[:ne] - w0[:ne], ref[:ne] - w0[:ne])) > 1e-6
+
+
+@pytest.mark.parametrize("case,compact", [("strat3d", False), ("strat3d", True),
+                                          ("strat2d", False), ("riemann3", True)])
+def test_resistive_stage(dev_mod, oracle_mod, case, compact):
+    """one advect1 stage with resistive sources (constant eta) against the
+    oracle; and the resistive dt limit of getdt_special"""
+    if case == "strat3d":
+        prob = problems.stratified(n=12, nxyz=(12, 10, 8), amp=3e-2)
+    elif case == "strat2d":
+        prob = _to_2d(problems.stratified(n=12, nxyz=(14, 12, 1), amp=3e-2))
+    else:
+        prob = problems.riemann_tube(96, {"rho": 1.0, "p": 0.8, "v": [0.1, -0.2, 0.3],
+                                          "b": [0.6, 0.5, -0.4]},
+                                     {"rho": 0.3, "p": 0.2, "v": [-0.1, 0.2, 0.0],
+                                      "b": [0.6, -0.7, 0.2]})
+    prob.eqpar[1] = 3e-3
+    o = make_oracle(prob, oracle_mod)
+    o.set_resistive(compact, 0.4)
+    w0 = o.get_w()
+    wct = perturbed(type(prob)(**{**prob.__dict__, "w": w0.copy()}), seed=3, amp=1e-3).w
+    wct = o.getboundary_on(wct)
+    ref = o.advect1(1e-3, wct, w0)
+    dev = dev_mod.Device(prob)
+    dev.set_resistive(compact, 0.4)
+    dev.upload(prob.w)
+    dev.getboundary()
+    dev.upload_slot(dev_mod.SLOT_S1, wct)
+    dev.upload_slot(dev_mod.SLOT_S2, w0)
+    dev.advect1(1e-3, dev_mod.SLOT_S1, dev_mod.SLOT_S2)
+    got = dev.download_slot(dev_mod.SLOT_S2)
+    ne = problems.var_index(prob.ndir)["nevolve"]
+    errs = per_var_errors(prob, got[:ne] - w0[:ne], ref[:ne] - w0[:ne])
+    assert max(errs) <= 1e-11, errs
+    # the dt limit before and after getdt_visc (which sets dtdiffpar=0.25)
+    assert dev.getdt_special(1e30) == pytest.approx(o.getdt_special(o.getdt_res(1e30)), rel=1e-13)
+    assert dev.getdt_special(1e30) == pytest.approx(o.getdt_special(o.getdt_res(1e30)), rel=1e-13)
+    dev.close()
+    # the sources are really on
+    prob0 = type(prob)(**{**prob.__dict__, "eqpar": prob.eqpar.copy()})
+    prob0.eqpar[1] = 0.0
+    ref0 = make_oracle(prob0, oracle_mod).advect1(1e-3, wct, w0)
+    assert max(per_var_errors(prob, ref0[:ne] - w0[:ne], ref[:ne] - w0[:ne])) > 1e-8
+
+
+def test_resistive_free_steps(dev_mod, oracle_mod):
+    """free-running steps with eta > 0 (the resistive dt limit binds) against
+    the oracle"""
+    prob = problems.stratified(n=12, nxyz=(12, 10, 8), amp=3e-2)
+    prob.eqpar[1] = 1.0  # 0.5 (1/12)^2 / 1 = 3.5e-3, below the courant dt
+    o = make_oracle(prob, oracle_mod)
+    o.set_resistive(False, 0.5)
+    rdts = [o.step(prob.courantpar, "fourstep") for _ in range(5)]
+    dev = dev_mod.Device(prob)
+    dev.set_resistive(False, 0.5)
+    dev.upload(prob.w)
+    dev.getboundary()
+    gdts = [dev.step(prob.courantpar) for _ in range(5)]
+    got = dev.download()
+    dev.close()
+    np.testing.assert_allclose(gdts, rdts, rtol=1e-12)
+    assert_close_state(prob, got, o.get_w(), TOL, 8, "resistive")
