@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) SAC_AXIS_OCC k_axis(AxisArgs a) {
 // kernel 1, 2-D and 3-D grids: LDS-staged line tiles of axis K (see line_tile)
 template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 __global__ void __launch_bounds__(line::NT) SAC_LINE_OCC k_line(AxisArgs a, int seg) {
-  __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::RR * line::W];
+  __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::Tile<K>::RR * line::Tile<K>::W];
   C numax = 0;
   line::line_tile<NDIM, NDIR, K, S, C, LEARNED>(a, seg, L, numax);
   if (a.visc_on) {
