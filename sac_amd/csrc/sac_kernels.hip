@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) SAC_AXIS_OCC k_axis(AxisArgs a) {
 // kernel 1, 2-D and 3-D grids: LDS-staged line tiles of axis K (see line_tile)
 template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 __global__ void __launch_bounds__(line::NT) SAC_LINE_OCC k_line(AxisArgs a, int seg) {
-  __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::Tile<K>::RR * line::Tile<K>::W];
+  __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::Tile<K, C>::RR * line::Tile<K, C>::W];
   C numax = 0;
   line::line_tile<NDIM, NDIR, K, S, C, LEARNED>(a, seg, L, numax);
   if (a.visc_on) {
@@ -105,10 +105,10 @@ __global__ void __launch_bounds__(line::NT) SAC_LINE_OCC k_line(AxisArgs a, int 
 
 // segment length along K: whole lines unless that leaves fewer than 2048
 // blocks (then halves, down to 32 cells)
-template <int NDIM, int K>
+template <int NDIM, int K, typename C>
 static int line_segment(const GridDesc& g) {
   const int ext = g.Mhi[K] - g.Mlo[K] + 1;
-  const long long lines = line::nblocks<NDIM, K>(g, ext);
+  const long long lines = line::nblocks<NDIM, K, C>(g, ext);
   int nseg = 1;
   while (lines * nseg < 2048 && ext / (2 * nseg) >= 32) nseg *= 2;
   return (ext + nseg - 1) / nseg;
@@ -116,8 +116,8 @@ static int line_segment(const GridDesc& g) {
 
 template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 static void launch_line(const AxisArgs& A, hipStream_t st) {
-  const int seg = line_segment<NDIM, K>(A.g);
-  long long nb = line::nblocks<NDIM, K>(A.g, seg);
+  const int seg = line_segment<NDIM, K, C>(A.g);
+  long long nb = line::nblocks<NDIM, K, C>(A.g, seg);
   nb = (nb + 15) / 16 * 16;  // tile pairing over the XCDs (see line_tile)
   if (nb > 0)
     hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C, LEARNED>), dim3((unsigned)nb), dim3(line::NT),

@@ -37,14 +37,16 @@ namespace line {
 constexpr int HALO = 3;  // reach of the setnu stencil
 constexpr int NT = 256;  // threads per block
 
-// Tile shape per axis: CH points per chunk along K, W lines per block
-// (CH * W = NT).  Along x (K = 0) a chunk is a run of the row itself, and
-// 32-point chunks of 8 rows measured 19 % faster than 16 x 16 at 256^3
-// (fewer barriers and ring refills per cell); along y and z, 16 lines x 16
-// points keeps a whole 128-byte line of each row in one tile row.
-template <int K>
+// Tile shape per axis and storage type: CH points per chunk along K, W
+// lines per block (CH * W = NT).  Along x (K = 0) a chunk is a run of the
+// row itself; long runs mean fewer barriers and ring refills per cell, and
+// at 256^3 the fastest were 64 points x 4 rows in FP64 (0.97 ms against
+// 1.28 for 16 x 16) and 128 x 2 in FP32 storage (0.56 against 0.89).  Along
+// y and z, 16 lines x 16 points keeps a 128-byte line of each row in one
+// tile row.
+template <int K, typename C>
 struct Tile {
-  static constexpr int CH = (K == 0) ? 32 : 16;
+  static constexpr int CH = (K == 0) ? (sizeof(C) == 8 ? 64 : 128) : 16;
   static constexpr int W = NT / CH;
   static constexpr int RR = CH + 2 * HALO;  // ring rows
 };
@@ -64,9 +66,9 @@ struct Slots {
   static constexpr int NV = BT + 1;
 };
 
-template <int K>
+template <int K, typename C>
 __device__ __forceinline__ int at(int v, int row, int col) {
-  constexpr int W = Tile<K>::W, RR = Tile<K>::RR;
+  constexpr int W = Tile<K, C>::W, RR = Tile<K, C>::RR;
   return (K == 0) ? (v * W + col) * RR + row : (v * RR + row) * W + col;
 }
 
@@ -88,10 +90,10 @@ __host__ __device__ inline int tile_origin(const GridDesc& g) {
 }
 
 // blocks of a launch for segments of `seg` cells along K
-template <int NDIM, int K>
+template <int NDIM, int K, typename C>
 __host__ __device__ inline long long nblocks(const GridDesc& g, int seg) {
   constexpr int A = Axes<NDIM, K>::A, B = Axes<NDIM, K>::B;
-  constexpr int W = Tile<K>::W;
+  constexpr int W = Tile<K, C>::W;
   const long long nA = (g.Mhi[A] - tile_origin<A>(g) + 1 + W - 1) / W;
   const long long nB = (B >= 0) ? (g.Mhi[B] - g.Mlo[B] + 1) : 1;
   const long long nK = (g.Mhi[K] - g.Mlo[K] + 1 + seg - 1) / seg;
@@ -106,7 +108,7 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
   using SL = Slots<NDIM, NDIR>;
   constexpr int NE = SL::NE, NP = SL::NP;
   constexpr int A = Axes<NDIM, K>::A, B = Axes<NDIM, K>::B;
-  constexpr int CH = Tile<K>::CH, W = Tile<K>::W, RR = Tile<K>::RR;
+  constexpr int CH = Tile<K, C>::CH, W = Tile<K, C>::W, RR = Tile<K, C>::RR;
   const GridDesc& g = a.g;
   const long long fs = g.fs, sk = g.st[K];
   const S* __restrict__ Wf = (const S*)a.wct;
@@ -129,7 +131,7 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
     const long long x = b % 8, r = b / 8;
     b = 2 * (x + 8 * (r / 2)) + (r % 2);
   }
-  if (b >= nblocks<NDIM, K>(g, seg)) return;
+  if (b >= nblocks<NDIM, K, C>(g, seg)) return;
   const int nA = (g.Mhi[A] - tile_origin<A>(g) + 1 + W - 1) / W;
   const int bA = (int)(b % nA);
   b /= nA;
@@ -236,17 +238,17 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
       C F[NE], v[NDIM];
       flux_pt<NDIR, NDIM, C>(pt, K, gamma, F, v);
 #pragma unroll
-      for (int n = 0; n < NE; n++) L[at<K>(SL::F0 + n, r0, iw)] = F[n];
+      for (int n = 0; n < NE; n++) L[at<K, C>(SL::F0 + n, r0, iw)] = F[n];
 #pragma unroll
       for (int l = 0; l < NDIM; l++) {
-        L[at<K>(SL::V0 + l, r0, iw)] = v[l];
-        L[at<K>(SL::B0 + l, r0, iw)] = (C)pt.bg[l];
-        L[at<K>(SL::M0 + l, r0, iw)] = pt.m[l];
+        L[at<K, C>(SL::V0 + l, r0, iw)] = v[l];
+        L[at<K, C>(SL::B0 + l, r0, iw)] = (C)pt.bg[l];
+        L[at<K, C>(SL::M0 + l, r0, iw)] = pt.m[l];
       }
-      L[at<K>(SL::PB, r0, iw)] = (C)ptotal_bg<NDIR, C>(pt, gamma);
+      L[at<K, C>(SL::PB, r0, iw)] = (C)ptotal_bg<NDIR, C>(pt, gamma);
       const double rtd = (double)pt.rho + pt.rhob;
-      L[at<K>(SL::RT, r0, iw)] = (C)rtd;
-      L[at<K>(SL::BT, r0, iw)] = (C)((double)pt.b[K] + pt.bg[K]);
+      L[at<K, C>(SL::RT, r0, iw)] = (C)rtd;
+      L[at<K, C>(SL::BT, r0, iw)] = (C)((double)pt.b[K] + pt.bg[K]);
       {
 #pragma clang fp contract(off)
         // q of setnu: rho', thermal energy (in FP64), v_l, b_l (l != K)
@@ -258,14 +260,14 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
           sb += bc * bc;
           sm += mc * mc;
         }
-        L[at<K>(SL::Q0 + 0, r0, iw)] = pt.rho;
-        L[at<K>(SL::Q0 + 1, r0, iw)] = (C)((double)pt.e - 0.5 * (sb + sm * inv));
+        L[at<K, C>(SL::Q0 + 0, r0, iw)] = pt.rho;
+        L[at<K, C>(SL::Q0 + 1, r0, iw)] = (C)((double)pt.e - 0.5 * (sb + sm * inv));
 #pragma unroll
-        for (int l = 0; l < NDIM; l++) L[at<K>(SL::Q0 + 2 + l, r0, iw)] = pt.m[l] * (C)inv;
+        for (int l = 0; l < NDIM; l++) L[at<K, C>(SL::Q0 + 2 + l, r0, iw)] = pt.m[l] * (C)inv;
 #pragma unroll
         for (int n = 0; n < NDIM - 1; n++) {
           const int lb = (n < K) ? n : n + 1;
-          L[at<K>(SL::Q0 + 2 + NDIM + n, r0, iw)] = pt.b[lb];
+          L[at<K, C>(SL::Q0 + 2 + NDIM + n, r0, iw)] = pt.b[lb];
         }
       }
     }
@@ -288,20 +290,20 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
 #pragma unroll
         for (int n = 0; n < NE; n++) {
           C E1, e2;
-          const C D1 = two_diff(L[at<K>(SL::F0 + n, rp1, iw)], L[at<K>(SL::F0 + n, rm1, iw)], E1);
-          const C D2 = two_diff(L[at<K>(SL::F0 + n, rp2, iw)], L[at<K>(SL::F0 + n, rm2, iw)], e2);
+          const C D1 = two_diff(L[at<K, C>(SL::F0 + n, rp1, iw)], L[at<K, C>(SL::F0 + n, rm1, iw)], E1);
+          const C D2 = two_diff(L[at<K, C>(SL::F0 + n, rp2, iw)], L[at<K, C>(SL::F0 + n, rm2, iw)], e2);
           R[n] = ((D2 - (C)8 * D1) + (e2 - (C)8 * E1)) * inv12;
         }
-        const C pTb0 = L[at<K>(SL::PB, r00, iw)];
-        const C bgk = L[at<K>(SL::B0 + K, r00, iw)];
+        const C pTb0 = L[at<K, C>(SL::PB, r00, iw)];
+        const C bgk = L[at<K, C>(SL::B0 + K, r00, iw)];
         C re = 0;
 #pragma unroll
         for (int l = 0; l < NDIM; l++) {
-          const C dv1 = L[at<K>(SL::V0 + l, rp1, iw)] - L[at<K>(SL::V0 + l, rm1, iw)];
-          const C dv2 = L[at<K>(SL::V0 + l, rp2, iw)] - L[at<K>(SL::V0 + l, rm2, iw)];
+          const C dv1 = L[at<K, C>(SL::V0 + l, rp1, iw)] - L[at<K, C>(SL::V0 + l, rm1, iw)];
+          const C dv2 = L[at<K, C>(SL::V0 + l, rp2, iw)] - L[at<K, C>(SL::V0 + l, rm2, iw)];
           const C Dv = -(dv2 - (C)8 * dv1) * inv12;
           if (l == K) re -= Dv * pTb0;
-          re += bgk * L[at<K>(SL::B0 + l, r00, iw)] * Dv;
+          re += bgk * L[at<K, C>(SL::B0 + l, r00, iw)] * Dv;
         }
         R[1 + NDIR] += re;
 #pragma unroll
@@ -312,8 +314,8 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
         //   m -= (b+bg) g_K,  b -= m/rho_t g_K,  e -= m.(b+bg)/rho_t g_K
         // with the state of the cell (loaded here: all ndir components)
         if (a.divbfix && valid && fluxc) {
-          const C D1b = L[at<K>(SL::BT, rp1, iw)] - L[at<K>(SL::BT, rm1, iw)];
-          const C D2b = L[at<K>(SL::BT, rp2, iw)] - L[at<K>(SL::BT, rm2, iw)];
+          const C D1b = L[at<K, C>(SL::BT, rp1, iw)] - L[at<K, C>(SL::BT, rm1, iw)];
+          const C D2b = L[at<K, C>(SL::BT, rp2, iw)] - L[at<K, C>(SL::BT, rm2, iw)];
           const C gK = -((D2b - (C)8 * D1b) * inv12);
           const double rt = (double)Wf[icell] + Bg[fs + icell];
           double sm = 0.0;
@@ -340,14 +342,14 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
           const int rp1 = row(1), rm1 = row(-1);
           const C h = (C)(0.5 / dxd);
           const bool inP = c + 1 <= MhiK, inM = c - 1 >= MloK;
-          const C rtp = L[at<K>(SL::RT, rp1, iw)], rtm = L[at<K>(SL::RT, rm1, iw)];
+          const C rtp = L[at<K, C>(SL::RT, rp1, iw)], rtm = L[at<K, C>(SL::RT, rm1, iw)];
 #pragma unroll
           for (int k = 0; k < K; k++) {
             const C Tmp = inP ? Tc[4 * k + 0] : (C)0, Tmm = inM ? Tc[4 * k + 1] : (C)0;
             const C Tbp = inP ? Tc[4 * k + 2] : (C)0, Tbm = inM ? Tc[4 * k + 3] : (C)0;
-            const C mkp = L[at<K>(SL::M0 + k, rp1, iw)], mkm = L[at<K>(SL::M0 + k, rm1, iw)];
-            const C bkp = L[at<K>(SL::Q0 + 2 + NDIM + k, rp1, iw)];
-            const C bkm = L[at<K>(SL::Q0 + 2 + NDIM + k, rm1, iw)];
+            const C mkp = L[at<K, C>(SL::M0 + k, rp1, iw)], mkm = L[at<K, C>(SL::M0 + k, rm1, iw)];
+            const C bkp = L[at<K, C>(SL::Q0 + 2 + NDIM + k, rp1, iw)];
+            const C bkm = L[at<K, C>(SL::Q0 + 2 + NDIM + k, rm1, iw)];
             R[1 + k] += (rtp * Tmp - rtm * Tmm) * h;
             R[1 + NDIR] += (mkp * Tmp - mkm * Tmm) * h;
             R[2 + NDIR + k] -= (Tbp - Tbm) * h;
@@ -359,14 +361,14 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
 #pragma clang fp contract(off)
         const bool z1 = (c == MloK && !g.per_lo[K]) || (c == MhiK && !g.per_hi[K]);
         const int rw[7] = {row(-3), row(-2), row(-1), row(0), row(1), row(2), row(3)};
-        const C rtm = L[at<K>(SL::RT, rw[2], iw)], rt0 = L[at<K>(SL::RT, rw[3], iw)],
-                rtp = L[at<K>(SL::RT, rw[4], iw)];
+        const C rtm = L[at<K, C>(SL::RT, rw[2], iw)], rt0 = L[at<K, C>(SL::RT, rw[3], iw)],
+                rtp = L[at<K, C>(SL::RT, rw[4], iw)];
         const C rhoL = (rt0 + rtm) * (C)0.5, rhoR = (rtp + rt0) * (C)0.5;
 #pragma unroll
         for (int p = 0; p < NP; p++) {
           C q[7];
 #pragma unroll
-          for (int s = 0; s < 7; s++) q[s] = L[at<K>(SL::Q0 + p, rw[s], iw)];
+          for (int s = 0; s < 7; s++) q[s] = L[at<K, C>(SL::Q0 + p, rw[s], iw)];
           const C ch = (p >= 2 && p < 2 + NDIM) ? (C)0.4 : (C)0.04;
           const C Kc = ctk * ch;
           C ratL, ratR;
@@ -390,9 +392,9 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
             R[1 + NDIR] += (tR - tL) * idx;
           } else if (p < 2 + NDIM) {
             const int l = p - 2;
-            const C m0 = L[at<K>(SL::M0 + l, rw[3], iw)];
-            const C mL = (m0 + L[at<K>(SL::M0 + l, rw[2], iw)]) * (C)0.5;
-            const C mR = (L[at<K>(SL::M0 + l, rw[4], iw)] + m0) * (C)0.5;
+            const C m0 = L[at<K, C>(SL::M0 + l, rw[3], iw)];
+            const C mL = (m0 + L[at<K, C>(SL::M0 + l, rw[2], iw)]) * (C)0.5;
+            const C mR = (L[at<K, C>(SL::M0 + l, rw[4], iw)] + m0) * (C)0.5;
             const C mult = (l == K) ? (C)1 : (C)0.5;  // the i==k branch runs twice for l==k
             R[1 + l] += mult * (rhoR * tR - rhoL * tL) * idx;
             R[1 + NDIR] += mult * (mR * tR - mL * tL) * idx;
