@@ -32,22 +32,11 @@
 #define SAC_LINE_TILES 1
 #endif
 
-// SAC_AXIS_WAVES: if set, the minimum number of waves per SIMD k_axis is
-// compiled for (the register budget follows: 512/waves VGPRs)
-#ifndef SAC_AXIS_WAVES
-#define SAC_AXIS_WAVES 0
-#endif
-#if SAC_AXIS_WAVES > 0
-#define SAC_AXIS_OCC __attribute__((amdgpu_waves_per_eu(SAC_AXIS_WAVES)))
-#else
-#define SAC_AXIS_OCC
-#endif
-
 namespace sac {
 
 // kernel 1: line groups of axis K (see axis_group)
 template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
-__global__ void __launch_bounds__(256) SAC_AXIS_OCC k_axis(AxisArgs a) {
+__global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
   constexpr int CK = LineGroup<K>::CK;
   const GridDesc& g = a.g;
   const long long t = xcd_block(a.xcd) * blockDim.x + threadIdx.x;
@@ -81,19 +70,9 @@ __global__ void __launch_bounds__(256) SAC_AXIS_OCC k_axis(AxisArgs a) {
   }
 }
 
-// SAC_LINE_WAVES: if set, the minimum waves per SIMD k_line is compiled for
-#ifndef SAC_LINE_WAVES
-#define SAC_LINE_WAVES 0
-#endif
-#if SAC_LINE_WAVES > 0
-#define SAC_LINE_OCC __attribute__((amdgpu_waves_per_eu(SAC_LINE_WAVES)))
-#else
-#define SAC_LINE_OCC
-#endif
-
 // kernel 1, 2-D and 3-D grids: LDS-staged line tiles of axis K (see line_tile)
 template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
-__global__ void __launch_bounds__(line::NT) SAC_LINE_OCC k_line(AxisArgs a, int seg) {
+__global__ void __launch_bounds__((line::Tile<K, C>::T)) k_line(AxisArgs a, int seg) {
   __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::Tile<K, C>::RR * line::Tile<K, C>::W];
   C numax = 0;
   line::line_tile<NDIM, NDIR, K, S, C, LEARNED>(a, seg, L, numax);
@@ -120,7 +99,8 @@ static void launch_line(const AxisArgs& A, hipStream_t st) {
   long long nb = line::nblocks<NDIM, K, C>(A.g, seg);
   nb = (nb + 15) / 16 * 16;  // tile pairing over the XCDs (see line_tile)
   if (nb > 0)
-    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C, LEARNED>), dim3((unsigned)nb), dim3(line::NT),
+    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C, LEARNED>), dim3((unsigned)nb),
+                       dim3(line::Tile<K, C>::T),
                        0, st, A, seg);
 }
 

@@ -43,11 +43,12 @@ constexpr int NT = 256;  // threads per block
 // at 256^3 the fastest were 64 points x 4 rows in FP64 (0.97 ms against
 // 1.28 for 16 x 16) and 128 x 2 in FP32 storage (0.56 against 0.89).  Along
 // y and z, 16 lines x 16 points keeps a 128-byte line of each row in one
-// tile row.
+// tile row (32-point chunks in 512-thread blocks were slower in FP64).
 template <int K, typename C>
 struct Tile {
   static constexpr int CH = (K == 0) ? (sizeof(C) == 8 ? 64 : 128) : 16;
-  static constexpr int W = NT / CH;
+  static constexpr int W = (K == 0) ? NT / CH : 16;
+  static constexpr int T = CH * W;          // threads per block
   static constexpr int RR = CH + 2 * HALO;  // ring rows
 };
 
