@@ -9,12 +9,13 @@
 // The arithmetic of each value is the same; the schedule differs.
 //
 // Schedule.  A block of 256 threads owns W neighbouring lines and marches
-// along K in chunks of CH points (16 x 16 along y and z, 8 rows x 32 points
-// along x; see Tile).  Every thread loads ONE new
-// point per chunk (13 fields, coalesced along x), evaluates everything that
-// depends on that point alone (the flux and velocities along K, p_Tb and
-// the background field, rho_t, momenta and the seven setnu q values) once,
-// and stores it in a ring of CH+6 rows in LDS.  After a barrier the thread
+// along K in chunks of CH points (16 x 16 along y and z; along x 4 rows of
+// 64 points in FP64, 2 rows of 128 in FP32 storage; see Tile).  Every
+// thread loads ONE new point per chunk (13 fields, coalesced along x),
+// evaluates everything that depends on that point alone (the flux and
+// velocities along K, p_Tb and the background field, rho_t, momenta and
+// the seven setnu q values) once, and stores it in a ring of CH+6 rows in
+// LDS.  After a barrier the thread
 // assembles the cell three points behind its point from the ring: the
 // flux differences of points c-2..c+2 and the face ratios of the 7-point q
 // window c-3..c+3.  Each point is loaded from HBM and evaluated once per
@@ -22,7 +23,7 @@
 //
 // Ring layout: [value][row][line] for K > 0 (a wave reads 4 rows x 16
 // consecutive lines, free of LDS bank conflicts), [value][line][row] for
-// K = 0 (a wave reads 32 consecutive ring rows of 2 lines).
+// K = 0 (a wave reads 64 consecutive ring rows of one line).
 //
 // Grid edges along K follow axis_group: the q layer beyond the grid is the
 // mirror of the cell 6 inward (src/vacusr.viscosity.t:378-400); points past
