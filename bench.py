@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the tube-ensemble leg (4096 uneven 1-D tubes to t=0.1)")
     ap.add_argument("--tubes", type=int, default=4096)
+    ap.add_argument("--no-patches", action="store_true",
+                    help="skip the 2-D patch-ensemble leg (1024 uneven patches to t=0.1)")
+    ap.add_argument("--patches", type=int, default=1024)
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-closure form (train + run at amplitude 0.1)")
     ap.add_argument("--learned-steps", type=int, default=12)
@@ -197,6 +200,8 @@ def main():
                          "bytes_per_cell_stage": BYTES_PER_CELL_STAGE["mixed"]}}}
     if not args.no_batch:
         out.setdefault("forms", {})["batched_tubes"] = batched_leg(args.tubes)
+    if not args.no_patches:
+        out.setdefault("forms", {})["batched_patches_2d"] = patches_leg(args.patches)
     if not args.no_learned:
         out.setdefault("forms", {})["learned_closure"] = learned_leg(n, args.learned_steps)
     if not args.no_cpu_baseline:
@@ -240,6 +245,48 @@ def batched_leg(count):
             "failed": int((st == 2).sum()),
             "wall_roofline_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4),
             "bitwise_equal_to_single_tube_runs": checks}
+
+
+def patches_leg(count):
+    """The 2-D patch ensemble: `count` 2-D MHD Riemann patches of 32-160
+    cells per axis (seeded), each run to t=0.1 with its own dt; autotuned
+    tile shape, then hipGraph replays of 32 steps until every patch is done.
+    Value = sum over patches of cells x steps x 4 stages / wall time of the
+    run.  The two smallest patches are run again alone (an ensemble of one)
+    and compared bit for bit."""
+    import numpy as np
+    from sac_amd import batch
+    patches = batch.patch_ensemble(count=count)
+    ens = batch.Ensemble(patches, courantpar=0.3, tmax=0.1)
+    ens.tune(trial_steps=4)
+    ck, tx, ty = ens.shape()
+    t0 = time.perf_counter()
+    issued = ens.run(steps_per_graph=32)
+    el = time.perf_counter() - t0
+    t, it, st = ens.status()
+    ws = ens.download()
+    ens.close()
+    cells = np.array([p.nx[0] * p.nx[1] for p in patches], dtype=np.int64)
+    cu = float(np.sum(cells * it)) * 4
+    checks = []
+    for i in np.argsort(cells)[:2]:
+        one = batch.Ensemble([patches[i]], courantpar=0.3, tmax=0.1)
+        one.run(steps_per_graph=32)
+        _, it1, _ = one.status()
+        w1 = one.download()[0]
+        one.close()
+        checks.append(bool(it1[0] == it[i] and np.array_equal(ws[i][:8], w1[:8])))
+    alg = BYTES_PER_CELL_STAGE["fp64"] * cu
+    return {"value": round(cu / el / 1e6, 1), "unit": "Mcell-updates/s",
+            "workload": f"{count} 2-D MHD Riemann patches, 32-160 x 32-160 cells (seed 2048), "
+                        "gamma 5/3, cont boundaries, RK4 + hyperdiffusion, to t=0.1, FP64",
+            "mesh_cells": int(cells.sum()), "wall_s": round(el, 3), "steps_issued": int(issued),
+            "steps_per_patch": [int(it.min()), int(it.max())],
+            "tuned_tile": {"x": tx, "y": ty},
+            "reached_tmax": int((st == 1).sum()), "stopped_dt_below_dtmin": int((st == 3).sum()),
+            "failed": int((st == 2).sum()),
+            "wall_roofline_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4),
+            "bitwise_equal_to_single_patch_runs": checks}
 
 
 def learned_leg(n, steps):

@@ -213,11 +213,11 @@ int sac_closure_grad(sac_handle h, const float *feat, const float *targ, int nsa
 int sac_closure_enable(sac_handle h, int on, double sensor);
 
 /* ------------------------------------------------------------------------
- * Ensembles of independent 1-D tubes (ndim=1, ndir=3, nw=13, 'cont'
- * boundaries, 'fourstep').  No reference counterpart as one call: each tube
- * is one serial run of the reference's main loop (src/vac.t:55-106) with its
- * own dt (getdt_courant + getdt_visc), t and it, stopping at the first
- * t >= tmax (or it >= itmax).  The ensemble advances on the device in
+ * Ensembles of independent 1-D tubes or 2-D patches (ndim=1 or 2, ndir=3,
+ * nw=13, 'cont' boundaries, 'fourstep', no gravity).  No reference
+ * counterpart as one call: each tube is one serial run of the reference's
+ * main loop (src/vac.t:55-106) with its own dt (getdt_courant +
+ * getdt_visc), t and it, stopping at the first t >= tmax (or it >= itmax).  The ensemble advances on the device in
  * hipGraph replays of several steps.  A tube stops at t >= tmax or it >= itmax
  * (state 1), at dt < dtmin = 1e-99 (state 3, the reference's third finish
  * condition, reached when a tube blows up), or when dt is not a number
@@ -232,11 +232,24 @@ typedef void *sac_batch_handle;
 int sac_batch_init(int ntubes, const int *ncell, int nw, const double *w, const double *dx,
                    const double *eqpar, int neqpar, int nu_, double courantpar, double tmax,
                    int itmax, sac_batch_handle *out);
+/* The same for tubes (ndim=1) or patches (ndim=2): ncell[ntubes*ndim] mesh
+ * cells per tube and axis (>= 6 each); w per tube w(G1[,G2], nw)
+ * column-major (G = ncell+4); dx per tube the G1 widths along x, then (2-D)
+ * the G2 widths along y.  At most 65535 patches.  sac_batch_init is this
+ * call with ndim=1. */
+int sac_batch_init_nd(int ndim, int ntubes, const int *ncell, int nw, const double *w,
+                      const double *dx, const double *eqpar, int neqpar, int nu_,
+                      double courantpar, double tmax, int itmax, sac_batch_handle *out);
 int sac_batch_free(sac_batch_handle h);
-/* Autotuner: times trial_steps steps for every launch shape (cells per
- * thread ck in {1,2,4} x threads per block tb in {64,128,256}) from the
- * current state, restores the state, keeps the fastest shape. */
+/* Autotuner: times trial_steps steps for every launch shape from the
+ * current state, restores the state, keeps the fastest shape.  Tubes: cells
+ * per thread ck in {1,2,4} x threads per block tb in {64,128,256}.
+ * Patches: one cell per thread, tiles of tbx x tb/tbx cells for tb in
+ * {64,128,256} and tbx in {16,...,tb} (12 shapes). */
 int sac_batch_tune(sac_batch_handle h, int trial_steps, int *ck, int *tb, double *ms_per_step);
+/* The current tile: cells per thread (ck) and the tile's extent along x and
+ * y in cells (tubes: tbx = threads*ck, tby = 1; patches: ck = 1). */
+int sac_batch_shape(sac_batch_handle h, int *ck, int *tbx, int *tby);
 /* Advance every tube to its end, replaying a graph of steps_per_graph steps
  * at most max_steps steps in total; returns the steps issued. */
 int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,

@@ -1,6 +1,7 @@
 """ctypes binding of the tube-ensemble entry points of libsac_amd.so
 (sac_batch_*, include/sac_amd.h): many independent 1-D MHD tubes of uneven
-length advanced together on one GPU, each with its own dt, t and it.
+length, or 2-D MHD patches of uneven size, advanced together on one GPU,
+each with its own dt, t and it.
 """
 from __future__ import annotations
 
@@ -12,8 +13,9 @@ from . import problems
 from .device import SacError, _dp, _ip, grid_dx, load_library
 
 EXPORTS = [
-    "sac_batch_init", "sac_batch_free", "sac_batch_tune", "sac_batch_run", "sac_batch_status",
-    "sac_batch_download", "sac_batch_memory", "sac_batch_last_error",
+    "sac_batch_init", "sac_batch_init_nd", "sac_batch_free", "sac_batch_tune", "sac_batch_shape",
+    "sac_batch_run", "sac_batch_status", "sac_batch_download", "sac_batch_memory",
+    "sac_batch_last_error",
 ]
 
 STATE = {0: "running", 1: "finished", 2: "failed", 3: "dt below dtmin"}
@@ -31,6 +33,8 @@ def _lib():
         llp = ctypes.POINTER(ctypes.c_longlong)
         sig = {
             "sac_batch_init": (I, [I, ip, I, dp, dp, dp, I, I, D, D, I, ctypes.POINTER(V)]),
+            "sac_batch_init_nd": (I, [I, I, ip, I, dp, dp, dp, I, I, D, D, I, ctypes.POINTER(V)]),
+            "sac_batch_shape": (I, [V, ip, ip, ip]),
             "sac_batch_free": (I, [V]),
             "sac_batch_tune": (I, [V, I, ip, ip, dp]),
             "sac_batch_run": (I, [V, I, ctypes.c_longlong, llp]),
@@ -58,24 +62,34 @@ def riemann_ensemble(count=4096, nmin=200, nmax=2000, seed=4096, nu=1.0):
             for n, l, r in problems.tube_ensemble(count, nmin, nmax, seed)]
 
 
+def patch_ensemble(count=1024, nmin=32, nmax=160, seed=2048, nu=1.0):
+    """The seeded 2-D patch ensemble as Problems (gamma 5/3, cont boundaries)."""
+    return [problems.riemann_patch(nx, ny, l, r, gamma=5.0 / 3.0, nu=nu)
+            for nx, ny, l, r in problems.patch_ensemble(count, nmin, nmax, seed)]
+
+
 class Ensemble:
-    """Independent 1-D tubes (ndim=1, ndir=3) on one device."""
+    """Independent 1-D tubes or 2-D patches (ndir=3) on one device."""
 
     def __init__(self, tubes, courantpar=0.3, tmax=0.1, itmax=1000000):
         L = _lib()
         self.tubes = tubes
-        self.ncell = np.ascontiguousarray([t.nx[0] for t in tubes], dtype=np.int32)
+        ndim = tubes[0].ndim
         nw = tubes[0].nw
         for t in tubes:
-            if t.ndim != 1 or t.ndir != 3:
-                raise SacError("tube ensembles are 1-D with ndir=3")
-        # per tube w(G, nw) column-major == prob.w[v, 0, 0, i] in C order
+            if t.ndim != ndim or ndim not in (1, 2) or t.ndir != 3:
+                raise SacError("an ensemble holds 1-D tubes or 2-D patches, all with ndir=3")
+        self.ndim = ndim
+        self.ncell = np.ascontiguousarray([t.nx[d] for t in tubes for d in range(ndim)],
+                                          dtype=np.int32)
+        # per tube w(G1[,G2], nw) column-major == prob.w in C order
         w = np.concatenate([np.ascontiguousarray(t.w.reshape(nw, -1)).ravel() for t in tubes])
         dx = np.concatenate([grid_dx(t) for t in tubes])
         eq = np.ascontiguousarray(tubes[0].eqpar, dtype=np.float64)
         h = ctypes.c_void_p()
-        _check(L.sac_batch_init(len(tubes), _ip(self.ncell), nw, _dp(w), _dp(dx), _dp(eq),
-                                len(eq), problems.NU, courantpar, tmax, itmax, ctypes.byref(h)))
+        _check(L.sac_batch_init_nd(ndim, len(tubes), _ip(self.ncell), nw, _dp(w), _dp(dx),
+                                   _dp(eq), len(eq), problems.NU, courantpar, tmax, itmax,
+                                   ctypes.byref(h)))
         self._h = h
         self.nw = nw
 
@@ -96,6 +110,13 @@ class Ensemble:
                                      ctypes.byref(ms)))
         return ck.value, tb.value, ms.value
 
+    def shape(self):
+        """Current tile: (cells per thread, extent along x, extent along y)."""
+        ck, tx, ty = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib().sac_batch_shape(self._h, ctypes.byref(ck), ctypes.byref(tx),
+                                      ctypes.byref(ty)))
+        return ck.value, tx.value, ty.value
+
     def run(self, steps_per_graph=32, max_steps=10 ** 7):
         n = ctypes.c_longlong()
         _check(_lib().sac_batch_run(self._h, steps_per_graph, max_steps, ctypes.byref(n)))
@@ -111,7 +132,7 @@ class Ensemble:
 
     def download(self):
         """Final state of every tube, shaped like its Problem's w."""
-        total = int(sum((n + 4) * self.nw for n in self.ncell))
+        total = int(sum(t.w.size for t in self.tubes))
         w = np.zeros(total)
         _check(_lib().sac_batch_download(self._h, _dp(w)))
         out, off = [], 0

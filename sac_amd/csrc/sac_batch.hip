@@ -1,22 +1,25 @@
-// sac_batch.hip -- ensembles of independent 1-D MHD tubes of uneven length
-// advanced together on one GPU.
+// sac_batch.hip -- ensembles of independent 1-D MHD tubes of uneven length,
+// or of 2-D MHD patches of uneven size, advanced together on one GPU.
 //
-// Every tube is its own serial run of the reference (src/vac.t:55-106): its
-// own dt from getdt_courant and getdt_special (lagged maxviscoef, forced
-// dtdiffpar 0.25), its own t and it, 'cont' boundaries, classical RK4
-// ('fourstep').  The tubes share the kernels: one launch per kernel covers
-// every tube through a tile table (tube, first mesh cell), tiles never
-// straddle two tubes, so the per-tube reductions are block reductions plus
-// one atomic per block.  The time loop runs on the device: a small kernel
-// computes each tube's next dt and finish flag, the RK stages use it, and a
-// second small kernel advances t and it.  One step is a fixed sequence of
-// launches, captured once in a hipGraph holding several steps and replayed
-// until every tube has reached tmax; finished tubes skip their tiles.
+// Every tube (or patch) is its own serial run of the reference
+// (src/vac.t:55-106): its own dt from getdt_courant and getdt_special
+// (lagged maxviscoef, forced dtdiffpar 0.25), its own t and it, 'cont'
+// boundaries, classical RK4 ('fourstep').  The tubes share the kernels: one
+// launch per kernel covers every tube through a tile table (tube, first
+// mesh cell along x and y), tiles never straddle two tubes, so the per-tube
+// reductions are block reductions plus one atomic per block.  The time loop
+// runs on the device: a small kernel computes each tube's next dt and
+// finish flag, the RK stages use it, and a second small kernel advances t
+// and it.  One step is a fixed sequence of launches, captured once in a
+// hipGraph holding several steps and replayed until every tube has reached
+// tmax; finished tubes skip their tiles.
 //
 // The line work and the combining update are the grid path's own device
-// functions (axis_group, final_cell) applied to a per-tube 1-D grid
-// descriptor, so a tube advanced here follows the same arithmetic as the
-// same tube run alone through sac_dev_*.
+// functions (axis_group, final_cell) applied to a per-tube grid descriptor,
+// so a 1-D tube advanced here follows the same arithmetic as the same tube
+// run alone through sac_dev_*.  A 2-D patch adds the per-axis kernels of
+// both axes and the cross-term factors T, as the grid path's line-group
+// kernels (k_axis) do.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,8 +40,9 @@ namespace batch {
 constexpr int NDIR = 3;
 constexpr int NE = 2 + 2 * NDIR;  // evolving fields
 constexpr int NBG = 2 + NDIR;     // background fields
+constexpr int NT2 = 4;            // cross-term factor fields of a 2-D patch (2*ndim*(ndim-1))
 constexpr int kSlots = 4;         // W, S1, S2, ACC
-constexpr int kRed = 2;           // per tube and register: max cmax, max cmax/dx
+constexpr int kRed = 4;           // per tube and register: max cmax_d, then max cmax_d/dx_d (d < ndim)
 
 // FINISHED: t >= tmax or it >= itmax; STOPPED: dt < dtmin (the reference's
 // third finish condition, src/vac.t:591, hit when a tube's state blows up
@@ -47,27 +51,35 @@ enum TubeState : int { RUNNING = 0, FINISHED = 1, FAILED = 2, STOPPED = 3 };
 
 struct Tile {
   int tube;  // tube index
-  int c0;    // first mesh cell (grid index) of the tile
+  int c0;    // first mesh cell (grid index) of the tile along x
+  int c1;    // and along y (2-D patches; 0 for tubes)
 };
 
 struct Args {
+  int ndim;                  // 1: tubes, 2: patches
   int ntubes;
   long long fs;              // field stride (elements)
   const long long* start;    // grid origin of each tube in the packed layout
-  const int* ncell;          // mesh cells per tube
-  const Tile* tiles;
+  const int* nxy;            // mesh cells per tube along x and y: [tube][2]
+  const int* pitch;          // row pitch (elements) of each tube's grid
+  const long long* dxo;      // offset of each tube's cell widths in dx (x, then y)
+  const Tile* tiles;         // tiles of the tubes in `active`
   int ntiles;
+  int tbx;                   // tile width along x in cells (2-D)
+  const int* active;         // tubes still running when the tiles were built
+  int nactive;
   double* reg[kSlots];
   const double* bg;
-  const double* dx;          // cell widths, packed like one field
+  const double* dx;          // cell widths per tube: G_x values, then G_y values
   double* R;
+  double* T;                 // cross-term factors (2-D), packed like NT2 fields
   unsigned long long* red;   // [tube][slot][kRed]
   unsigned long long* visc;  // [tube] maxviscoef since the last dt
   double* t;
   double* dt;
   int* it;
   int* state;
-  const double* dxmin;       // per tube, over its mesh
+  const double* dxmin;       // per tube and axis, over its mesh: [tube][2]
   double gamma;
   int visc_on;
   double courantpar, tmax, dtmin;
@@ -75,17 +87,24 @@ struct Args {
   unsigned int* running;     // count of tubes still running after a step
 };
 
+template <int NDIM>
 __device__ __forceinline__ GridDesc tube_grid(const Args& a, int tube) {
   GridDesc g{};
-  g.ndim = 1;
+  g.ndim = NDIM;
   g.ndir = NDIR;
-  const int n = a.ncell[tube];
-  g.G[0] = n + 4;
-  g.G[1] = g.G[2] = 1;
+  const int nx = a.nxy[2 * tube];
+  g.G[0] = nx + 4;
+  g.G[1] = (NDIM == 2) ? a.nxy[2 * tube + 1] + 4 : 1;
+  g.G[2] = 1;
   g.Mlo[0] = 2;
-  g.Mhi[0] = n + 1;
+  g.Mhi[0] = nx + 1;
+  if (NDIM == 2) {
+    g.Mlo[1] = 2;
+    g.Mhi[1] = g.G[1] - 3;
+  }
   g.st[0] = 1;
-  g.st[1] = g.st[2] = g.G[0];
+  g.st[1] = a.pitch[tube];
+  g.st[2] = g.st[1] * g.G[1];
   g.fs = a.fs;
   return g;
 }
@@ -94,28 +113,47 @@ __device__ __forceinline__ unsigned long long* red_of(const Args& a, int tube, i
   return a.red + ((size_t)tube * kSlots + slot) * kRed;
 }
 
-// line work of one tile: each thread one group of CK cells
-template <int CK>
+// cell of thread `tid`, cell j of its group of CK along x (tubes), or the
+// tile position (patches, CK = 1); false outside the mesh
+template <int NDIM, int CK>
+__device__ __forceinline__ bool tile_cell(const Args& a, const Tile& tl, const GridDesc& g, int j,
+                                          int (&c)[3]) {
+  const int tid = (int)threadIdx.x;
+  if (NDIM == 1) {
+    c[0] = tl.c0 + tid * CK + j;
+    c[1] = c[2] = 0;
+    return c[0] <= g.Mhi[0];
+  }
+  c[0] = tl.c0 + tid % a.tbx;
+  c[1] = tl.c1 + tid / a.tbx;
+  c[2] = 0;
+  return c[0] <= g.Mhi[0] && c[1] <= g.Mhi[1];
+}
+
+// line work of axis K for one tile: each thread one group of CK cells
+template <int NDIM, int CK, int K>
 __global__ void __launch_bounds__(256) kb_axis(Args a, int wct) {
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
-  const GridDesc g = tube_grid(a, tl.tube);
+  const GridDesc g = tube_grid<NDIM>(a, tl.tube);
   const long long o = a.start[tl.tube];
   AxisArgs x{};
   x.g = g;
   x.prec = PREC_FP64;
   x.wct = a.reg[wct] + o;
   x.bg = a.bg + o;
-  x.dx[0] = a.dx + o;
+  x.dx[0] = a.dx + a.dxo[tl.tube];
+  if (NDIM == 2) x.dx[1] = x.dx[0] + g.G[0];
   x.R = a.R + o;
-  x.T = nullptr;
+  x.T = (NDIM == 2) ? a.T + o : nullptr;
   x.gamma = a.gamma;
   x.visc_on = a.visc_on;
   x.ctot_src = red_of(a, tl.tube, wct);
   x.red_visc = a.visc + tl.tube;
   double numax = 0.0;
-  int c[3] = {tl.c0 + (int)threadIdx.x * CK, 0, 0};
-  if (c[0] <= g.Mhi[0]) axis_group<1, NDIR, 0, CK, double, double>(x, g, c, numax);
+  int c[3];
+  if (tile_cell<NDIM, CK>(a, tl, g, 0, c))
+    axis_group<NDIM, NDIR, K, CK, double, double>(x, g, c, numax);
   if (a.visc_on) {
     double r[1] = {numax};
     block_max_atomic<1>(r, x.red_visc);
@@ -123,12 +161,12 @@ __global__ void __launch_bounds__(256) kb_axis(Args a, int wct) {
 }
 
 // combining update of one tile: out = base + dt/da * R, acc_out = acc_in + dt/db * R
-template <int CK>
+template <int NDIM, int CK>
 __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int out, double da,
                                                 int acc_in, int acc_out, double db) {
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;
-  const GridDesc g = tube_grid(a, tl.tube);
+  const GridDesc g = tube_grid<NDIM>(a, tl.tube);
   const long long o = a.start[tl.tube];
   const double dt = a.dt[tl.tube];
   FinalArgs f{};
@@ -136,9 +174,10 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
   f.prec = PREC_FP64;
   f.wct = a.reg[wct] + o;
   f.bg = a.bg + o;
-  f.dx[0] = a.dx + o;
+  f.dx[0] = a.dx + a.dxo[tl.tube];
+  if (NDIM == 2) f.dx[1] = f.dx[0] + g.G[0];
   f.R = a.R + o;
-  f.T = nullptr;
+  f.T = (NDIM == 2) ? a.T + o : nullptr;
   f.base = a.reg[base] + o;
   f.out = a.reg[out] + o;
   f.acc_in = acc_in >= 0 ? a.reg[acc_in] + o : nullptr;
@@ -147,54 +186,94 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
   f.qb = acc_out >= 0 ? dt / db : 0.0;
   f.gamma = a.gamma;
   f.visc_on = a.visc_on;
-  double redv[2] = {0.0, 0.0};
+  double redv[2 * NDIM];
+#pragma unroll
+  for (int k = 0; k < 2 * NDIM; k++) redv[k] = 0.0;
 #pragma unroll
   for (int j = 0; j < CK; j++) {
-    int c[3] = {tl.c0 + (int)threadIdx.x * CK + j, 0, 0};
-    if (c[0] <= g.Mhi[0]) {
-      double r[2] = {0.0, 0.0};
-      final_cell<1, NDIR, double, double>(f, g, c, r);
-      redv[0] = fmax(redv[0], r[0]);
-      redv[1] = fmax(redv[1], r[1]);
+    int c[3];
+    if (tile_cell<NDIM, CK>(a, tl, g, j, c)) {
+      double r[2 * NDIM];
+      final_cell<NDIM, NDIR, double, double>(f, g, c, r);
+#pragma unroll
+      for (int k = 0; k < 2 * NDIM; k++) redv[k] = fmax(redv[k], r[k]);
     }
   }
-  block_max_atomic<2>(redv, red_of(a, tl.tube, out));
+  block_max_atomic<2 * NDIM>(redv, red_of(a, tl.tube, out));
 }
 
 // cmax reductions of register `slot` of every running tube
-template <int CK>
+template <int NDIM, int CK>
 __global__ void __launch_bounds__(256) kb_cmax(Args a, int slot) {
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;
-  const GridDesc g = tube_grid(a, tl.tube);
+  const GridDesc g = tube_grid<NDIM>(a, tl.tube);
   const long long o = a.start[tl.tube];
-  double redv[2] = {0.0, 0.0};
+  const double* dx0 = a.dx + a.dxo[tl.tube];
+  double redv[2 * NDIM];
+#pragma unroll
+  for (int k = 0; k < 2 * NDIM; k++) redv[k] = 0.0;
 #pragma unroll
   for (int j = 0; j < CK; j++) {
-    const int c0 = tl.c0 + (int)threadIdx.x * CK + j;
-    if (c0 <= g.Mhi[0]) {
+    int c[3];
+    if (tile_cell<NDIM, CK>(a, tl, g, j, c)) {
       Pt<NDIR, double> p;
-      load_pt<NDIR, double, double>(a.reg[slot] + o, a.bg + o, a.fs, c0, p);
-      double cm[1];
-      cmax_all<1, NDIR>(p, a.gamma, cm);
-      redv[0] = fmax(redv[0], cm[0]);
-      redv[1] = fmax(redv[1], cm[0] / a.dx[o + c0]);
+      load_pt<NDIR, double, double>(a.reg[slot] + o, a.bg + o, a.fs, lin(g, c[0], c[1], c[2]), p);
+      double cm[NDIM];
+      cmax_all<NDIM, NDIR>(p, a.gamma, cm);
+#pragma unroll
+      for (int d = 0; d < NDIM; d++) {
+        const double dxd = (d == 0) ? dx0[c[0]] : dx0[g.G[0] + c[1]];
+        redv[d] = fmax(redv[d], cm[d]);
+        redv[NDIM + d] = fmax(redv[NDIM + d], cm[d] / dxd);
+      }
     }
   }
-  block_max_atomic<2>(redv, red_of(a, tl.tube, slot));
+  block_max_atomic<2 * NDIM>(redv, red_of(a, tl.tube, slot));
 }
 
 // 'cont' ghost cells of nfield packed fields f: 2 per side, copies of the
 // edge cell (getboundary, src/vacgrid.t:366-370)
 __global__ void kb_ghost(Args a, double* f, int nfield) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int tube = (int)(i >> 2), k = (int)(i & 3);
-  if (tube >= a.ntubes || a.state[tube] != RUNNING) return;
-  const int n = a.ncell[tube];
+  const int k = (int)(i & 3);
+  if ((i >> 2) >= a.nactive) return;
+  const int tube = a.active[i >> 2];
+  if (a.state[tube] != RUNNING) return;
+  const int n = a.nxy[2 * tube];
   const long long o = a.start[tube];
   const int cg = (k < 2) ? k : n + k;  // ghost cells 0,1 and n+2,n+3
   const int ce = (k < 2) ? 2 : n + 1;  // edge mesh cell
   for (int v = 0; v < nfield; v++) f[v * a.fs + o + cg] = f[v * a.fs + o + ce];
+}
+
+// 'cont' ghost cells of a 2-D patch (blockIdx.y = entry of the active list).  The reference
+// fills the x regions over the whole grid along y, then the y regions over
+// the whole grid along x (src/vacgrid.t:195-512), so every ghost cell ends
+// up with the mesh cell at its clamped indices.  The ghost frame of a
+// G0 x G1 grid: rows 0, 1, G1-2, G1-1 in full (4*G0 cells), then columns
+// 0, 1, G0-2, G0-1 of the mesh rows (4*(G1-4) cells).
+__global__ void kb_ghost2(Args a, double* f, int nfield) {
+  const int tube = a.active[blockIdx.y];
+  if (a.state[tube] != RUNNING) return;
+  const int nx = a.nxy[2 * tube], ny = a.nxy[2 * tube + 1];
+  const int G0 = nx + 4, G1 = ny + 4;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= 4 * G0 + 4 * ny) return;
+  int c0, c1;
+  if (i < 4 * G0) {
+    const int r = i / G0;
+    c0 = i - r * G0;
+    c1 = (r < 2) ? r : G1 - 4 + r;
+  } else {
+    const int j = i - 4 * G0, k = j & 3;
+    c1 = 2 + (j >> 2);
+    c0 = (k < 2) ? k : G0 - 4 + k;
+  }
+  const int e0 = min(max(c0, 2), nx + 1), e1 = min(max(c1, 2), ny + 1);
+  const long long p = a.pitch[tube];
+  const long long dst = a.start[tube] + c0 + c1 * p, src = a.start[tube] + e0 + e1 * p;
+  for (int v = 0; v < nfield; v++) f[v * a.fs + dst] = f[v * a.fs + src];
 }
 
 // zero the reductions of register `slot` of every tube
@@ -217,13 +296,17 @@ __device__ __forceinline__ double red_val(const unsigned long long* p) {
 __global__ void kb_dt(Args a) {
   const int tube = blockIdx.x * blockDim.x + threadIdx.x;
   if (tube >= a.ntubes || a.state[tube] != RUNNING) return;
-  const double cm = red_val(red_of(a, tube, SAC_SLOT_W) + 1);
+  const unsigned long long* r = red_of(a, tube, SAC_SLOT_W);
+  double cm = 0.0;
+  for (int d = 0; d < a.ndim; d++) cm = fmax(cm, red_val(r + a.ndim + d));
   double dt = 1.0e99;
   if (cm > 1.0e-99) dt = fmin(dt, a.courantpar / cm);
   if (a.visc_on) {
     const double mv = red_val(a.visc + tube);
-    const double dxm = a.dxmin[tube];
-    dt = fmin(dt, 0.25 / (mv / (dxm * dxm)));
+    for (int d = 0; d < a.ndim; d++) {
+      const double dxm = a.dxmin[2 * tube + d];
+      dt = fmin(dt, 0.25 / (mv / (dxm * dxm)));
+    }
     a.visc[tube] = 0ull;
   }
   if (a.it[tube] >= a.itmax || a.t[tube] >= a.tmax) {
@@ -273,30 +356,43 @@ int bfail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return bfail(SAC_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-// launch shapes the autotuner chooses from: cells per thread (CK) and
-// threads per block (TB); a tile is TB*CK consecutive cells of one tube
+// launch shapes the autotuner chooses from.  Tubes: cells per thread (ck)
+// and threads per block (tb), a tile being tb*ck consecutive cells of one
+// tube.  Patches: one cell per thread, tiles of tbx x (tb/tbx) cells.
 struct Shape {
-  int ck, tb;
+  int ck, tb, tbx;
 };
-const Shape kShapes[] = {{1, 64}, {1, 128}, {1, 256}, {2, 64}, {2, 128},
-                         {2, 256}, {4, 64}, {4, 128}, {4, 256}};
+const Shape kShapes1[] = {{1, 64, 0},  {1, 128, 0}, {1, 256, 0}, {2, 64, 0},  {2, 128, 0},
+                          {2, 256, 0}, {4, 64, 0},  {4, 128, 0}, {4, 256, 0}};
+const Shape kShapes2[] = {{1, 64, 16},  {1, 64, 32},  {1, 64, 64},   {1, 128, 16},
+                          {1, 128, 32}, {1, 128, 64}, {1, 128, 128}, {1, 256, 16},
+                          {1, 256, 32}, {1, 256, 64}, {1, 256, 128}, {1, 256, 256}};
 
 struct Batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  int ndim = 1;
   int ntubes = 0;
-  std::vector<int> ncell;
+  std::vector<int> nxy;          // [tube][2] mesh cells along x and y (1 along y for tubes)
+  std::vector<int> pitch;        // row pitch of each tube's grid
   std::vector<long long> start;  // packed grid origins
+  std::vector<long long> dxo;    // offsets of each tube's cell widths
+  std::vector<int> active;       // tubes the tiles cover (running ones after a compaction)
   long long fs = 0;
+  int max_frame = 0;             // ghost cells of the largest patch
   Args a{};
   // device buffers
   long long* d_start = nullptr;
-  int* d_ncell = nullptr;
+  int* d_nxy = nullptr;
+  int* d_pitch = nullptr;
+  long long* d_dxo = nullptr;
   Tile* d_tiles = nullptr;
+  int* d_active = nullptr;
   double* d_reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   double* d_bg = nullptr;
   double* d_dx = nullptr;
   double* d_R = nullptr;
+  double* d_T = nullptr;
   double* d_save = nullptr;  // state kept aside while the autotuner runs
   unsigned long long* d_red = nullptr;
   unsigned long long* d_visc = nullptr;
@@ -306,12 +402,15 @@ struct Batch {
   int* d_state = nullptr;
   double* d_dxmin = nullptr;
   unsigned int* d_running = nullptr;
-  Shape shape{2, 128};
+  Shape shape{2, 128, 0};
   int ntiles = 0;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
   long long bytes = 0;
   long long steps_run = 0;
+
+  int gx(int i) const { return nxy[2 * i] + 4; }
+  int gy(int i) const { return ndim == 2 ? nxy[2 * i + 1] + 4 : 1; }
 };
 
 int balloc(Batch& b, void** p, size_t bytes) {
@@ -326,9 +425,17 @@ int balloc(Batch& b, void** p, size_t bytes) {
 
 int set_shape(Batch& b, Shape s) {
   std::vector<Tile> tiles;
-  const int span = s.ck * s.tb;
-  for (int i = 0; i < b.ntubes; i++)
-    for (int c = 0; c < b.ncell[i]; c += span) tiles.push_back({i, 2 + c});
+  if (b.ndim == 1) {
+    const int span = s.ck * s.tb;
+    for (int i : b.active)
+      for (int c = 0; c < b.nxy[2 * i]; c += span) tiles.push_back({i, 2 + c, 0});
+  } else {
+    const int tby = s.tb / s.tbx;
+    for (int i : b.active)
+      for (int c1 = 0; c1 < b.nxy[2 * i + 1]; c1 += tby)
+        for (int c0 = 0; c0 < b.nxy[2 * i]; c0 += s.tbx) tiles.push_back({i, 2 + c0, 2 + c1});
+  }
+  if (tiles.empty()) tiles.push_back({b.active.empty() ? 0 : b.active[0], 2, 2});
   if (b.d_tiles) hipFree(b.d_tiles);
   b.d_tiles = nullptr;
   BCHK(hipMalloc((void**)&b.d_tiles, sizeof(Tile) * tiles.size()));
@@ -337,6 +444,11 @@ int set_shape(Batch& b, Shape s) {
   b.shape = s;
   b.a.tiles = b.d_tiles;
   b.a.ntiles = b.ntiles;
+  b.a.tbx = s.tbx;
+  BCHK(hipMemcpy(b.d_active, b.active.data(), sizeof(int) * b.active.size(),
+                 hipMemcpyHostToDevice));
+  b.a.active = b.d_active;
+  b.a.nactive = (int)b.active.size();
   if (b.graph) {
     hipGraphExecDestroy(b.graph);
     b.graph = nullptr;
@@ -344,13 +456,23 @@ int set_shape(Batch& b, Shape s) {
   return 0;
 }
 
-template <int CK>
+void enqueue_ghost(Batch& b, double* f, int nfield) {
+  if (b.active.empty()) return;
+  if (b.ndim == 1) {
+    const dim3 gg((unsigned)((4LL * b.active.size() + 255) / 256));
+    hipLaunchKernelGGL(kb_ghost, gg, dim3(256), 0, b.stream, b.a, f, nfield);
+  } else {
+    const dim3 gg((unsigned)((b.max_frame + 255) / 256), (unsigned)b.active.size());
+    hipLaunchKernelGGL(kb_ghost2, gg, dim3(256), 0, b.stream, b.a, f, nfield);
+  }
+}
+
+template <int NDIM, int CK>
 void enqueue_step_ck(Batch& b) {
   const Args& a = b.a;
   hipStream_t st = b.stream;
   const dim3 gt(b.ntiles), bt(b.shape.tb);
   const dim3 gn((b.ntubes + 255) / 256), bn(256);
-  const dim3 gg((unsigned)((4LL * b.ntubes + 255) / 256));
   const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, A = SAC_SLOT_ACC;
   hipLaunchKernelGGL(kb_dt, gn, bn, 0, st, a);
   // (wct, base, out, da, acc_in, acc_out, db) of the four RK4 stages, as in
@@ -367,20 +489,25 @@ void enqueue_step_ck(Batch& b) {
                         {S2, W, S1, 1.0, A, A, 3.0},
                         {S1, A, W, 6.0, -1, -1, 0.0}};
   for (const St& s : stages) {
-    hipLaunchKernelGGL(kb_axis<CK>, gt, bt, 0, st, a, s.wct);
+    hipLaunchKernelGGL((kb_axis<NDIM, CK, 0>), gt, bt, 0, st, a, s.wct);
+    if (NDIM == 2) hipLaunchKernelGGL((kb_axis<NDIM, CK, NDIM - 1>), gt, bt, 0, st, a, s.wct);
     hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, st, a, s.out);
-    hipLaunchKernelGGL(kb_final<CK>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai, s.ao,
-                       s.db);
-    hipLaunchKernelGGL(kb_ghost, gg, dim3(256), 0, st, a, a.reg[s.out], NE);
+    hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
+                       s.ao, s.db);
+    enqueue_ghost(b, a.reg[s.out], NE);
   }
   hipLaunchKernelGGL(kb_tick, gn, bn, 0, st, a);
 }
 
 void enqueue_step(Batch& b) {
+  if (b.ndim == 2) {
+    enqueue_step_ck<2, 1>(b);
+    return;
+  }
   switch (b.shape.ck) {
-    case 1: enqueue_step_ck<1>(b); break;
-    case 2: enqueue_step_ck<2>(b); break;
-    default: enqueue_step_ck<4>(b); break;
+    case 1: enqueue_step_ck<1, 1>(b); break;
+    case 2: enqueue_step_ck<1, 2>(b); break;
+    default: enqueue_step_ck<1, 4>(b); break;
   }
 }
 
@@ -388,10 +515,14 @@ void enqueue_cmax(Batch& b, int slot) {
   const dim3 gn((b.ntubes + 255) / 256), bn(256);
   hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, b.stream, b.a, slot);
   const dim3 gt(b.ntiles), bt(b.shape.tb);
+  if (b.ndim == 2) {
+    hipLaunchKernelGGL((kb_cmax<2, 1>), gt, bt, 0, b.stream, b.a, slot);
+    return;
+  }
   switch (b.shape.ck) {
-    case 1: hipLaunchKernelGGL(kb_cmax<1>, gt, bt, 0, b.stream, b.a, slot); break;
-    case 2: hipLaunchKernelGGL(kb_cmax<2>, gt, bt, 0, b.stream, b.a, slot); break;
-    default: hipLaunchKernelGGL(kb_cmax<4>, gt, bt, 0, b.stream, b.a, slot); break;
+    case 1: hipLaunchKernelGGL((kb_cmax<1, 1>), gt, bt, 0, b.stream, b.a, slot); break;
+    case 2: hipLaunchKernelGGL((kb_cmax<1, 2>), gt, bt, 0, b.stream, b.a, slot); break;
+    default: hipLaunchKernelGGL((kb_cmax<1, 4>), gt, bt, 0, b.stream, b.a, slot); break;
   }
 }
 
@@ -413,15 +544,51 @@ int build_graph(Batch& b, int steps) {
   return 0;
 }
 
-// reset the per-tube run state (t, it, state, maxviscoef) from host values
-int reset_state(Batch& b, const std::vector<double>& t0) {
-  std::vector<int> zi(b.ntubes, 0);
-  BCHK(hipMemcpy(b.d_t, t0.data(), sizeof(double) * b.ntubes, hipMemcpyHostToDevice));
-  BCHK(hipMemcpy(b.d_it, zi.data(), sizeof(int) * b.ntubes, hipMemcpyHostToDevice));
-  BCHK(hipMemcpy(b.d_state, zi.data(), sizeof(int) * b.ntubes, hipMemcpyHostToDevice));
-  BCHK(hipMemset(b.d_visc, 0, sizeof(unsigned long long) * b.ntubes));
-  BCHK(hipMemset(b.d_dt, 0, sizeof(double) * b.ntubes));
+// per-tube run state (t, dt, it, state, maxviscoef), kept aside by the autotuner
+struct RunState {
+  std::vector<double> t, dt;
+  std::vector<int> it, state;
+  std::vector<unsigned long long> visc;
+};
+
+int save_state(Batch& b, RunState& r) {
+  const size_t n = b.ntubes;
+  r.t.resize(n);
+  r.dt.resize(n);
+  r.it.resize(n);
+  r.state.resize(n);
+  r.visc.resize(n);
+  BCHK(hipMemcpy(r.t.data(), b.d_t, sizeof(double) * n, hipMemcpyDeviceToHost));
+  BCHK(hipMemcpy(r.dt.data(), b.d_dt, sizeof(double) * n, hipMemcpyDeviceToHost));
+  BCHK(hipMemcpy(r.it.data(), b.d_it, sizeof(int) * n, hipMemcpyDeviceToHost));
+  BCHK(hipMemcpy(r.state.data(), b.d_state, sizeof(int) * n, hipMemcpyDeviceToHost));
+  BCHK(hipMemcpy(r.visc.data(), b.d_visc, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
   return 0;
+}
+
+int restore_state(Batch& b, const RunState& r) {
+  const size_t n = b.ntubes;
+  BCHK(hipMemcpy(b.d_t, r.t.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b.d_dt, r.dt.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b.d_it, r.it.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b.d_state, r.state.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b.d_visc, r.visc.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// drop finished tubes from the tiles (their blocks would only exit), then
+// recapture the graph; the arithmetic of the running tubes is unchanged
+int compact(Batch& b) {
+  std::vector<int> st(b.ntubes);
+  BCHK(hipMemcpy(st.data(), b.d_state, sizeof(int) * b.ntubes, hipMemcpyDeviceToHost));
+  std::vector<int> act;
+  for (int i : b.active)
+    if (st[i] == RUNNING) act.push_back(i);
+  if (act.empty() || act.size() == b.active.size()) return 0;
+  b.active.swap(act);
+  const int steps = b.graph_steps;
+  if (int rc = set_shape(b, b.shape)) return rc;
+  return build_graph(b, steps);
 }
 
 }  // namespace
@@ -430,20 +597,27 @@ extern "C" {
 
 const char* sac_batch_last_error(void) { return g_berr.c_str(); }
 
-int sac_batch_init(int ntubes, const int* ncell, int nw, const double* w, const double* dx,
-                   const double* eqpar, int neqpar, int nu_, double courantpar, double tmax,
-                   int itmax, sac_batch_handle* out) {
+int sac_batch_init_nd(int ndim, int ntubes, const int* ncell, int nw, const double* w,
+                      const double* dx, const double* eqpar, int neqpar, int nu_,
+                      double courantpar, double tmax, int itmax, sac_batch_handle* out) {
   g_berr.clear();
   if (!out) return bfail(SAC_E_ARG, "out is null");
   *out = nullptr;
+  if (ndim != 1 && ndim != 2)
+    return bfail(SAC_E_UNSUPPORTED, "ensembles are of 1-D tubes or 2-D patches (ndim=%d)", ndim);
   if (ntubes < 1 || !ncell || !w || !dx || !eqpar) return bfail(SAC_E_ARG, "bad arguments");
+  if (ndim == 2 && ntubes > 65535)
+    return bfail(SAC_E_ARG, "at most 65535 patches per ensemble (%d)", ntubes);
   if (nw != NE + NBG) return bfail(SAC_E_UNSUPPORTED, "tubes carry nw=%d variables (ndir=3)", NE + NBG);
   if (neqpar < 2 || eqpar[0] <= 0.0) return bfail(SAC_E_ARG, "gamma must be > 0");
   if (eqpar[1] != 0.0) return bfail(SAC_E_UNSUPPORTED, "resistive MHD (eta /= 0) is not built");
   if (!(courantpar > 0.0) || !(tmax > 0.0) || itmax < 1)
     return bfail(SAC_E_ARG, "courantpar, tmax and itmax must be positive");
   for (int i = 0; i < ntubes; i++)
-    if (ncell[i] < 6) return bfail(SAC_E_ARG, "tube %d has %d < 6 cells", i, ncell[i]);
+    for (int d = 0; d < ndim; d++)
+      if (ncell[ndim * i + d] < 6)
+        return bfail(SAC_E_ARG, "tube %d has %d < 6 cells along axis %d", i, ncell[ndim * i + d],
+                     d + 1);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return bfail(SAC_E_HIP, "no HIP device available");
@@ -453,13 +627,23 @@ int sac_batch_init(int ntubes, const int* ncell, int nw, const double* w, const 
     return code;
   };
   hipGetDevice(&b->device);
+  b->ndim = ndim;
   b->ntubes = ntubes;
-  b->ncell.assign(ncell, ncell + ntubes);
+  b->nxy.assign(2 * (size_t)ntubes, 1);
+  for (int i = 0; i < ntubes; i++)
+    for (int d = 0; d < ndim; d++) b->nxy[2 * i + d] = ncell[ndim * i + d];
   b->start.resize(ntubes);
-  long long off = 0;
+  b->pitch.resize(ntubes);
+  b->dxo.resize(ntubes);
+  long long off = 0, doff = 0;
   for (int i = 0; i < ntubes; i++) {
-    b->start[i] = off;
-    off += (ncell[i] + 4 + 15) / 16 * 16;  // each tube starts on a 128-byte line
+    const int G0 = b->gx(i), G1 = b->gy(i);
+    b->start[i] = off;  // each tube starts on a 128-byte line
+    b->pitch[i] = (ndim == 1) ? G0 : (G0 + 15) / 16 * 16;
+    off += ((long long)b->pitch[i] * G1 + 15) / 16 * 16;
+    b->dxo[i] = doff;
+    doff += G0 + (ndim == 2 ? G1 : 0);
+    b->max_frame = std::max(b->max_frame, 4 * G0 + 4 * (G1 - 4));
   }
   b->fs = (off + 63) / 64 * 64;
   if (hipStreamCreate(&b->stream) != hipSuccess) return bail(bfail(SAC_E_HIP, "hipStreamCreate"));
@@ -469,9 +653,12 @@ int sac_batch_init(int ntubes, const int* ncell, int nw, const double* w, const 
     if ((rc = balloc(*b, (void**)&b->d_reg[s], fb * NE))) return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_bg, fb * NBG))) return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_R, fb * NE))) return bail(rc);
-  if ((rc = balloc(*b, (void**)&b->d_dx, fb))) return bail(rc);
+  if (ndim == 2 && (rc = balloc(*b, (void**)&b->d_T, fb * NT2))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_dx, sizeof(double) * (size_t)doff))) return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_start, sizeof(long long) * ntubes))) return bail(rc);
-  if ((rc = balloc(*b, (void**)&b->d_ncell, sizeof(int) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_nxy, sizeof(int) * 2 * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_pitch, sizeof(int) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_dxo, sizeof(long long) * ntubes))) return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_red, sizeof(unsigned long long) * ntubes * kSlots * kRed)))
     return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_visc, sizeof(unsigned long long) * ntubes))) return bail(rc);
@@ -479,46 +666,62 @@ int sac_batch_init(int ntubes, const int* ncell, int nw, const double* w, const 
   if ((rc = balloc(*b, (void**)&b->d_dt, sizeof(double) * ntubes))) return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_it, sizeof(int) * ntubes))) return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_state, sizeof(int) * ntubes))) return bail(rc);
-  if ((rc = balloc(*b, (void**)&b->d_dxmin, sizeof(double) * ntubes))) return bail(rc);
+  if ((rc = balloc(*b, (void**)&b->d_dxmin, sizeof(double) * 2 * ntubes))) return bail(rc);
   if ((rc = balloc(*b, (void**)&b->d_running, sizeof(unsigned int)))) return bail(rc);
-  // host layout per tube: w(G_i, nw) column-major, tubes one after another;
-  // dx: G_i values per tube
-  std::vector<double> pk((size_t)b->fs * (NE + NBG), 0.0), pdx((size_t)b->fs, 1.0);
-  std::vector<double> dxmin(ntubes);
-  long long hoff = 0, hdx = 0;
+  if ((rc = balloc(*b, (void**)&b->d_active, sizeof(int) * ntubes))) return bail(rc);
+  b->active.resize(ntubes);
+  for (int i = 0; i < ntubes; i++) b->active[i] = i;
+  // host layout per tube: w(G0[,G1], nw) column-major, tubes one after
+  // another; dx: G0 values (then G1 values) per tube
+  std::vector<double> pk((size_t)b->fs * (NE + NBG), 0.0);
+  std::vector<double> dxmin(2 * (size_t)ntubes, 1.0);
+  long long hoff = 0;
   for (int i = 0; i < ntubes; i++) {
-    const int G = ncell[i] + 4;
+    const int G0 = b->gx(i), G1 = b->gy(i);
     for (int v = 0; v < NE + NBG; v++)
-      memcpy(&pk[(size_t)v * b->fs + b->start[i]], w + hoff + (size_t)v * G, sizeof(double) * G);
-    memcpy(&pdx[b->start[i]], dx + hdx, sizeof(double) * G);
-    double mn = 1.0e99;
-    for (int c = 2; c <= ncell[i] + 1; c++) {
-      if (!(dx[hdx + c] > 0.0)) return bail(bfail(SAC_E_ARG, "dx must be > 0 (tube %d)", i));
-      mn = std::min(mn, dx[hdx + c]);
+      for (int r = 0; r < G1; r++)
+        memcpy(&pk[(size_t)v * b->fs + b->start[i] + (size_t)r * b->pitch[i]],
+               w + hoff + ((size_t)v * G1 + r) * G0, sizeof(double) * G0);
+    for (int d = 0; d < ndim; d++) {
+      const double* dd = dx + b->dxo[i] + (d == 1 ? G0 : 0);
+      const int n = b->nxy[2 * i + d];
+      double mn = 1.0e99;
+      for (int c = 2; c <= n + 1; c++) {
+        if (!(dd[c] > 0.0)) return bail(bfail(SAC_E_ARG, "dx must be > 0 (tube %d)", i));
+        mn = std::min(mn, dd[c]);
+      }
+      dxmin[2 * i + d] = mn;
     }
-    dxmin[i] = mn;
-    hoff += (long long)G * (NE + NBG);
-    hdx += G;
+    hoff += (long long)G0 * G1 * (NE + NBG);
   }
   if (hipMemcpy(b->d_reg[SAC_SLOT_W], pk.data(), fb * NE, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_bg, pk.data() + (size_t)b->fs * NE, fb * NBG, hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMemcpy(b->d_dx, pdx.data(), fb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_dx, dx, sizeof(double) * (size_t)doff, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_start, b->start.data(), sizeof(long long) * ntubes, hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMemcpy(b->d_ncell, ncell, sizeof(int) * ntubes, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_dxmin, dxmin.data(), sizeof(double) * ntubes, hipMemcpyHostToDevice) !=
+      hipMemcpy(b->d_nxy, b->nxy.data(), sizeof(int) * 2 * ntubes, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_pitch, b->pitch.data(), sizeof(int) * ntubes, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_dxo, b->dxo.data(), sizeof(long long) * ntubes, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_dxmin, dxmin.data(), sizeof(double) * 2 * ntubes, hipMemcpyHostToDevice) !=
           hipSuccess)
     return bail(bfail(SAC_E_HIP, "upload failed"));
   Args& a = b->a;
+  a.ndim = ndim;
   a.ntubes = ntubes;
   a.fs = b->fs;
   a.start = b->d_start;
-  a.ncell = b->d_ncell;
+  a.nxy = b->d_nxy;
+  a.pitch = b->d_pitch;
+  a.dxo = b->d_dxo;
   for (int s = 0; s < kSlots; s++) a.reg[s] = b->d_reg[s];
   a.bg = b->d_bg;
   a.dx = b->d_dx;
   a.R = b->d_R;
+  a.T = b->d_T;
   a.red = b->d_red;
   a.visc = b->d_visc;
   a.t = b->d_t;
@@ -534,16 +737,22 @@ int sac_batch_init(int ntubes, const int* ncell, int nw, const double* w, const 
   a.dtmin = 1.0e-99;
   a.itmax = itmax;
   a.running = b->d_running;
-  if ((rc = set_shape(*b, Shape{2, 128}))) return bail(rc);
+  if ((rc = set_shape(*b, ndim == 1 ? Shape{2, 128, 0} : Shape{1, 256, 64}))) return bail(rc);
   // initial getboundary (background and state) and cmax of W
-  const dim3 gg((unsigned)((4LL * ntubes + 255) / 256));
-  hipLaunchKernelGGL(kb_ghost, gg, dim3(256), 0, b->stream, a, b->d_bg, NBG);
-  hipLaunchKernelGGL(kb_ghost, gg, dim3(256), 0, b->stream, a, b->d_reg[SAC_SLOT_W], NE);
+  enqueue_ghost(*b, b->d_bg, NBG);
+  enqueue_ghost(*b, b->d_reg[SAC_SLOT_W], NE);
   enqueue_cmax(*b, SAC_SLOT_W);
   if (hipStreamSynchronize(b->stream) != hipSuccess)
     return bail(bfail(SAC_E_HIP, "initial fill: %s", hipGetErrorString(hipGetLastError())));
   *out = b;
   return 0;
+}
+
+int sac_batch_init(int ntubes, const int* ncell, int nw, const double* w, const double* dx,
+                   const double* eqpar, int neqpar, int nu_, double courantpar, double tmax,
+                   int itmax, sac_batch_handle* out) {
+  return sac_batch_init_nd(1, ntubes, ncell, nw, w, dx, eqpar, neqpar, nu_, courantpar, tmax,
+                           itmax, out);
 }
 
 int sac_batch_free(sac_batch_handle h) {
@@ -555,10 +764,14 @@ int sac_batch_free(sac_batch_handle h) {
   hipFree(b->d_bg);
   hipFree(b->d_dx);
   hipFree(b->d_R);
+  hipFree(b->d_T);
   hipFree(b->d_save);
   hipFree(b->d_start);
-  hipFree(b->d_ncell);
+  hipFree(b->d_nxy);
+  hipFree(b->d_pitch);
+  hipFree(b->d_dxo);
   hipFree(b->d_tiles);
+  hipFree(b->d_active);
   hipFree(b->d_red);
   hipFree(b->d_visc);
   hipFree(b->d_t);
@@ -583,14 +796,18 @@ int sac_batch_tune(sac_batch_handle h, int trial_steps, int* ck, int* tb, double
     if (rc) return rc;
   }
   std::vector<unsigned long long> red0((size_t)b.ntubes * kSlots * kRed);
-  std::vector<double> t0(b.ntubes);
+  RunState r0;
   BCHK(hipMemcpy(b.d_save, b.d_reg[SAC_SLOT_W], fb, hipMemcpyDeviceToDevice));
   BCHK(hipMemcpy(red0.data(), b.d_red, sizeof(unsigned long long) * red0.size(),
                  hipMemcpyDeviceToHost));
-  BCHK(hipMemcpy(t0.data(), b.d_t, sizeof(double) * b.ntubes, hipMemcpyDeviceToHost));
+  if (int rc = save_state(b, r0)) return rc;
   double best = 1e300;
   Shape bs = b.shape;
-  for (const Shape& s : kShapes) {
+  const Shape* shapes = (b.ndim == 1) ? kShapes1 : kShapes2;
+  const int nshape = (b.ndim == 1) ? (int)(sizeof(kShapes1) / sizeof(Shape))
+                                   : (int)(sizeof(kShapes2) / sizeof(Shape));
+  for (int k = 0; k < nshape; k++) {
+    const Shape& s = shapes[k];
     if (int rc = set_shape(b, s)) return rc;
     if (int rc = build_graph(b, trial_steps)) return rc;
     BCHK(hipGraphLaunch(b.graph, b.stream));  // warm-up replay
@@ -615,12 +832,21 @@ int sac_batch_tune(sac_batch_handle h, int trial_steps, int* ck, int* tb, double
     BCHK(hipMemcpy(b.d_reg[SAC_SLOT_W], b.d_save, fb, hipMemcpyDeviceToDevice));
     BCHK(hipMemcpy(b.d_red, red0.data(), sizeof(unsigned long long) * red0.size(),
                    hipMemcpyHostToDevice));
-    if (int rc = reset_state(b, t0)) return rc;
+    if (int rc = restore_state(b, r0)) return rc;
   }
   if (int rc = set_shape(b, bs)) return rc;
   if (ck) *ck = bs.ck;
   if (tb) *tb = bs.tb;
   if (ms_per_step) *ms_per_step = best;
+  return 0;
+}
+
+int sac_batch_shape(sac_batch_handle h, int* ck, int* tbx, int* tby) {
+  if (!h) return bfail(SAC_E_ARG, "null handle");
+  const Batch& b = *(const Batch*)h;
+  if (ck) *ck = b.shape.ck;
+  if (tbx) *tbx = (b.ndim == 1) ? b.shape.tb * b.shape.ck : b.shape.tbx;
+  if (tby) *tby = (b.ndim == 1) ? 1 : b.shape.tb / b.shape.tbx;
   return 0;
 }
 
@@ -640,6 +866,10 @@ int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,
     BCHK(hipStreamSynchronize(b.stream));
     done += steps_per_graph;
     if (running == 0) break;
+    // once at most half of the covered tubes run, rebuild the tiles
+    if (2 * (size_t)running <= b.active.size()) {
+      if (int rc = compact(b)) return rc;
+    }
   }
   b.steps_run += done;
   if (steps_done) *steps_done = done;
@@ -665,10 +895,12 @@ int sac_batch_download(sac_batch_handle h, double* w) {
   BCHK(hipMemcpy(pk.data() + (size_t)b.fs * NE, b.d_bg, fb * NBG, hipMemcpyDeviceToHost));
   long long hoff = 0;
   for (int i = 0; i < b.ntubes; i++) {
-    const int G = b.ncell[i] + 4;
+    const int G0 = b.gx(i), G1 = b.gy(i);
     for (int v = 0; v < NE + NBG; v++)
-      memcpy(w + hoff + (size_t)v * G, &pk[(size_t)v * b.fs + b.start[i]], sizeof(double) * G);
-    hoff += (long long)G * (NE + NBG);
+      for (int r = 0; r < G1; r++)
+        memcpy(w + hoff + ((size_t)v * G1 + r) * G0,
+               &pk[(size_t)v * b.fs + b.start[i] + (size_t)r * b.pitch[i]], sizeof(double) * G0);
+    hoff += (long long)G0 * G1 * (NE + NBG);
   }
   return 0;
 }

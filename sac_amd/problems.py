@@ -201,3 +201,54 @@ def tube_ensemble(count=4096, nmin=200, nmax=2000, seed=4096):
             s["b"] = np.array([b1, s["b23"][0], s["b23"][1]])
         out.append((int(ns[i]), sides[0], sides[1]))
     return out
+
+
+def riemann_patch(nx, ny, left, right, gamma=5.0 / 3.0, nu=1.0):
+    """2-D MHD Riemann patch, d=23 (ndim=2, ndir=3, nw=13), zero background,
+    cont boundaries on [0,1]^2 with nx x ny cells, the two states split by
+    the diagonal x + y = 1 (left below it).  The in-plane field b1, b2 must
+    be the same on both sides (div B = 0 across the oblique front)."""
+    ndim, ndir = 2, 3
+    w, x, v = _alloc(ndim, ndir, (nx, ny))
+    xc, yc = _mesh_coords(2, (nx, ny))
+    X, Y = np.meshgrid(xc, yc)          # (ny, nx), x fastest
+    x[0, 0, 2:-2, 2:-2] = X
+    x[1, 0, 2:-2, 2:-2] = Y
+    L = X + Y <= 1.0
+    rho = np.where(L, left["rho"], right["rho"])
+    p = np.where(L, left["p"], right["p"])
+    vel = [np.where(L, left["v"][d], right["v"][d]) for d in range(3)]
+    b = [np.where(L, left["b"][d], right["b"][d]) for d in range(3)]
+    e = p / (gamma - 1.0) + 0.5 * (rho * sum(vv * vv for vv in vel) + sum(bb * bb for bb in b))
+    w[0, 0, 2:-2, 2:-2] = rho
+    for d in range(3):
+        w[v["m0"] + d, 0, 2:-2, 2:-2] = rho * vel[d]
+        w[v["b0"] + d, 0, 2:-2, 2:-2] = b[d]
+    w[v["e"] - 1, 0, 2:-2, 2:-2] = e
+    eq = np.zeros(NEQPAR)
+    eq[0] = gamma
+    eq[NU - 1] = nu
+    tb = np.full((v["nw"], 4), BC["cont"], dtype=np.int32)
+    return Problem("riemann_patch", ndim, ndir, (nx, ny), w, x, eq, tb,
+                   courantpar=0.3, tmax=0.05)
+
+
+def patch_ensemble(count=1024, nmin=32, nmax=160, seed=2048):
+    """Seeded parameters of an uneven 2-D patch ensemble: nx, ny ~
+    U{nmin..nmax} each; per side rho,p ~ U[0.1,1], v ~ U[-0.5,0.5]^3,
+    b3 ~ U[-1,1]; b1, b2 ~ U[0,1] shared by both sides."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns = rng.integers(nmin, nmax + 1, size=(count, 2))
+    out = []
+    for i in range(count):
+        sides = []
+        for _ in range(2):
+            rho, p = rng.uniform(0.1, 1.0, size=2)
+            vel = rng.uniform(-0.5, 0.5, size=3)
+            b3 = rng.uniform(-1.0, 1.0)
+            sides.append({"rho": rho, "p": p, "v": vel, "b3": b3})
+        b12 = rng.uniform(0.0, 1.0, size=2)
+        for s in sides:
+            s["b"] = np.array([b12[0], b12[1], s["b3"]])
+        out.append((int(ns[i, 0]), int(ns[i, 1]), sides[0], sides[1]))
+    return out

@@ -58,3 +58,23 @@ def test_init_rejects_unsupported_configurations():
     rc = L.sac_dev_init(1, 2, device._ip(nx), 11, device._ip(iws), 6, device._dp(eq), 6, 3, 6,
                         device._dp(dx), device._ip(tb), 0, 4, 0, ctypes.byref(h))
     assert rc == 1 and b"nw" in L.sac_dev_last_error()
+
+
+def test_batch_init_rejects_bad_ensembles():
+    """The ensemble entry point validates its arguments before any device call."""
+    import ctypes
+    import numpy as np
+    from sac_amd import batch
+    L = batch._lib()
+    h = ctypes.c_void_p()
+    eq = np.array([5.0 / 3.0, 0, 0, 0, 0, 1.0])
+    w = np.zeros(13 * 16 * 16)
+    dx = np.full(32, 0.1)
+    n = np.array([12, 12], dtype=np.int32)
+    args = lambda ndim, nc: (ndim, 1, batch._ip(nc), 13, batch._dp(w), batch._dp(dx),
+                             batch._dp(eq), 6, 6, 0.3, 0.1, 10, ctypes.byref(h))
+    assert L.sac_batch_init_nd(*args(3, n)) == 2
+    assert b"2-D patches" in L.sac_batch_last_error()
+    assert L.sac_batch_init_nd(*args(2, np.array([12, 5], dtype=np.int32))) == 1
+    assert b"axis 2" in L.sac_batch_last_error()
+    assert not h.value

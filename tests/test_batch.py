@@ -95,3 +95,172 @@ def test_ensemble_first_steps_match_oracle(mods, oracle_mod):
         assert t[i] == pytest.approx(sum(dts), rel=1e-8)
         errs = per_var_errors(prob, ws[i], o.get_w(), 8)
         assert max(errs) <= 1e-8, (i, errs)
+
+
+# ---------------------------------------------------------------------------
+# 2-D patches (sac_batch_init_nd with ndim=2)
+
+def _small_patches(batch, count=4, seed=5):
+    return batch.patch_ensemble(count=count, nmin=12, nmax=40, seed=seed)
+
+
+def test_patch_first_step_matches_oracle(mods, oracle_mod):
+    """One step of every patch against the oracle's step of the same patch.
+    The first dt has no viscous limit (maxviscoef is zero before the first
+    step), so both sides take the Courant dt of the initial state and the
+    states after the step must agree to the parity bar of 1e-12 per
+    variable: both axes' flux and hyperdiffusion terms, the cross terms of
+    the 2-D viscous sources and the ghost fill of the patch frame."""
+    batch, _ = mods
+    from tests.helpers import make_oracle, per_var_errors
+    patches = _small_patches(batch)
+    ens = batch.Ensemble(patches, tmax=1e9, itmax=1)
+    ens.run(steps_per_graph=1)
+    t, it, st = ens.status()
+    ws = ens.download()
+    ens.close()
+    assert (it == 1).all() and (st == 1).all(), (it, st)
+    for i, prob in enumerate(patches):
+        o = make_oracle(prob, oracle_mod)
+        dt = o.step(prob.courantpar)
+        assert t[i] == pytest.approx(dt, rel=1e-13)
+        errs = per_var_errors(prob, ws[i], o.get_w(), 8)
+        assert max(errs) <= 1e-12, (i, prob.nx, errs)
+
+
+def test_patch_steps_match_oracle_under_the_same_dt(mods, oracle_mod):
+    """Three steps of every patch against the oracle advanced with the dt
+    sequence the ensemble chose (recovered from its t after 1, 2 and 3
+    steps), to the parity bar of 1e-12.  The dt sequences themselves are not
+    compared: from the second step on dt comes from maxviscoef, whose setnu
+    ratios md3/md1 in nearly flat regions are ratios of rounding-level
+    differences, so any change of summation order moves it (the oracle and
+    the grid path's line tiles part by 1 % on one of these patches, while
+    the ensemble and the grid path agree)."""
+    batch, _ = mods
+    from tests.helpers import make_oracle, per_var_errors
+    patches = _small_patches(batch, count=3, seed=9)
+    ts = []
+    for itmax in (1, 2, 3):
+        ens = batch.Ensemble(patches, tmax=1e9, itmax=itmax)
+        ens.run(steps_per_graph=1)
+        t, it, _ = ens.status()
+        assert (it == itmax).all(), it
+        ts.append(t)
+        if itmax == 3:
+            ws = ens.download()
+        ens.close()
+    for i, prob in enumerate(patches):
+        o = make_oracle(prob, oracle_mod)
+        dts = [ts[0][i], ts[1][i] - ts[0][i], ts[2][i] - ts[1][i]]
+        for dt in dts:
+            o.advance(dt)
+        errs = per_var_errors(prob, ws[i], o.get_w(), 8)
+        assert max(errs) <= 1e-12, (i, prob.nx, errs)
+
+
+def test_patch_ensemble_matches_patches_run_alone(mods):
+    """Patches run to tmax together and each alone as an ensemble of one:
+    batching changes nothing, bit for bit (states, step counts, times)."""
+    batch, _ = mods
+    patches = _small_patches(batch, count=5, seed=13)
+    tmax = 0.01
+    ens = batch.Ensemble(patches, tmax=tmax)
+    ens.run(steps_per_graph=4)
+    t, it, st = ens.status()
+    ws = ens.download()
+    ens.close()
+    assert (st == 1).all(), st
+    for i, prob in enumerate(patches):
+        one = batch.Ensemble([prob], tmax=tmax)
+        one.run(steps_per_graph=8)
+        t1, it1, _ = one.status()
+        w1 = one.download()[0]
+        one.close()
+        assert it[i] == it1[0] and t[i] == t1[0], (i, it[i], it1[0])
+        np.testing.assert_array_equal(ws[i][:8], w1[:8], err_msg=f"patch {i}")
+
+
+def test_patch_ensemble_follows_single_patch_grid_runs(mods):
+    """Against the grid entry points (line-tile kernels, the same terms in
+    another summation order) run alone under the ensemble's dt sequence:
+    5 steps, states within 1e-12 per variable."""
+    batch, device = mods
+    from tests.helpers import per_var_errors
+    patches = _small_patches(batch, count=4, seed=21)
+    ts = []
+    for itmax in range(1, 6):
+        ens = batch.Ensemble(patches, tmax=1e9, itmax=itmax)
+        ens.run(steps_per_graph=itmax)
+        t, it, _ = ens.status()
+        assert (it == itmax).all(), it
+        ts.append(t)
+        if itmax == 5:
+            ws = ens.download()
+        ens.close()
+    for i, prob in enumerate(patches):
+        dev = device.Device(prob)
+        dev.upload(prob.w)
+        dev.getboundary()
+        prev = 0.0
+        for k in range(5):
+            dev.advance(ts[k][i] - prev)
+            prev = ts[k][i]
+        w1 = dev.download()
+        dev.close()
+        errs = per_var_errors(prob, ws[i], w1, 8)
+        assert max(errs) <= 1e-12, (i, prob.nx, errs)
+
+
+def test_patch_tuning_does_not_change_the_run(mods):
+    batch, _ = mods
+    patches = _small_patches(batch, count=5, seed=17)
+    a = batch.Ensemble(patches, tmax=0.005)
+    ck, tb, ms = a.tune(trial_steps=2)
+    ck2, tx, ty = a.shape()
+    assert ck == ck2 == 1 and tb in (64, 128, 256) and tx * ty == tb and ms > 0
+    a.run(steps_per_graph=4)
+    wa = a.download()
+    ta, ia, _ = a.status()
+    a.close()
+    b = batch.Ensemble(patches, tmax=0.005)
+    assert b.shape() == (1, 64, 4)
+    b.run(steps_per_graph=16)
+    wb = b.download()
+    tb_, ib, _ = b.status()
+    b.close()
+    np.testing.assert_array_equal(ia, ib)
+    np.testing.assert_array_equal(ta, tb_)
+    for x, y in zip(wa, wb):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_tuning_in_the_middle_of_a_run_keeps_the_run_state(mods):
+    """Tubes finish at different times, so a run compacts its tiles as it
+    goes; tuning after part of the run must restore every tube's t, it, dt,
+    state and maxviscoef, and finishing the run must give the uninterrupted
+    result bit for bit."""
+    batch, _ = mods
+    tubes = _small_ensemble(batch, count=8, seed=23)
+    ref = batch.Ensemble(tubes, tmax=0.02)
+    ref.run(steps_per_graph=4)
+    wr = ref.download()
+    tr, ir, sr = ref.status()
+    ref.close()
+    e = batch.Ensemble(tubes, tmax=0.02)
+    e.run(steps_per_graph=4, max_steps=int(np.median(ir)))
+    _, i_mid, s_mid = e.status()
+    assert (s_mid == 1).any() and (s_mid == 0).any(), s_mid  # some done, some running
+    e.tune(trial_steps=2)
+    _, i_after, s_after = e.status()
+    np.testing.assert_array_equal(i_mid, i_after)
+    np.testing.assert_array_equal(s_mid, s_after)
+    e.run(steps_per_graph=4)
+    we = e.download()
+    te, ie, se = e.status()
+    e.close()
+    np.testing.assert_array_equal(ie, ir)
+    np.testing.assert_array_equal(te, tr)
+    np.testing.assert_array_equal(se, sr)
+    for x, y in zip(we, wr):
+        np.testing.assert_array_equal(x, y)
