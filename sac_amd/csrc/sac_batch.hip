@@ -130,11 +130,15 @@ __device__ __forceinline__ bool tile_cell(const Args& a, const Tile& tl, const G
   return c[0] <= g.Mhi[0] && c[1] <= g.Mhi[1];
 }
 
-// line work of axis K for one tile: each thread one group of CK cells
+// line work of axis K for one tile: each thread one group of CK cells.  The
+// x kernel's first tile of a tube also clears the tube's reductions of
+// register `out`, which the stage's kb_final then fills (out != wct).
 template <int NDIM, int CK, int K>
-__global__ void __launch_bounds__(256) kb_axis(Args a, int wct) {
+__global__ void __launch_bounds__(256) kb_axis(Args a, int wct, int out) {
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
+  if (K == 0 && threadIdx.x < kRed && tl.c0 == 2 && (NDIM == 1 || tl.c1 == 2))
+    red_of(a, tl.tube, out)[threadIdx.x] = 0ull;
   const GridDesc g = tube_grid<NDIM>(a, tl.tube);
   const long long o = a.start[tl.tube];
   AxisArgs x{};
@@ -489,9 +493,9 @@ void enqueue_step_ck(Batch& b) {
                         {S2, W, S1, 1.0, A, A, 3.0},
                         {S1, A, W, 6.0, -1, -1, 0.0}};
   for (const St& s : stages) {
-    hipLaunchKernelGGL((kb_axis<NDIM, CK, 0>), gt, bt, 0, st, a, s.wct);
-    if (NDIM == 2) hipLaunchKernelGGL((kb_axis<NDIM, CK, NDIM - 1>), gt, bt, 0, st, a, s.wct);
-    hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, st, a, s.out);
+    hipLaunchKernelGGL((kb_axis<NDIM, CK, 0>), gt, bt, 0, st, a, s.wct, s.out);
+    if (NDIM == 2)
+      hipLaunchKernelGGL((kb_axis<NDIM, CK, NDIM - 1>), gt, bt, 0, st, a, s.wct, s.out);
     hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
                        s.ao, s.db);
     enqueue_ghost(b, a.reg[s.out], NE);
