@@ -164,7 +164,34 @@ __global__ void __launch_bounds__(256) kb_axis(Args a, int wct, int out) {
   }
 }
 
-// combining update of one tile: out = base + dt/da * R, acc_out = acc_in + dt/db * R
+// 'cont' ghost cells (getboundary, src/vacgrid.t:366-370) whose value is
+// that of mesh cell c of register f, written by the thread that stored c.
+// The reference fills the x regions over the whole grid along y, then the
+// y regions, so every ghost cell takes the mesh cell at its clamped
+// indices: c supplies the ghosts beyond each edge it lies on, including
+// the corners.
+template <int NDIM>
+__device__ __forceinline__ void cont_ghosts_of(const GridDesc& g, double* f, const int (&c)[3]) {
+  int gx[3] = {c[0], 0, 0}, gy[3] = {c[1], 0, 0};
+  int nxg = 1, nyg = 1;
+  if (c[0] == g.Mlo[0]) gx[nxg++] = 0, gx[nxg++] = 1;
+  if (c[0] == g.Mhi[0]) gx[nxg++] = g.Mhi[0] + 1, gx[nxg++] = g.Mhi[0] + 2;
+  if (NDIM == 2) {
+    if (c[1] == g.Mlo[1]) gy[nyg++] = 0, gy[nyg++] = 1;
+    if (c[1] == g.Mhi[1]) gy[nyg++] = g.Mhi[1] + 1, gy[nyg++] = g.Mhi[1] + 2;
+  }
+  if (nxg == 1 && nyg == 1) return;
+  const long long ic = lin(g, c[0], c[1], 0);
+  for (int n = 0; n < NE; n++) {
+    const double v = f[n * g.fs + ic];
+    for (int j = 0; j < nyg; j++)
+      for (int i = 0; i < nxg; i++)
+        if (i || j) f[n * g.fs + lin(g, gx[i], gy[j], 0)] = v;
+  }
+}
+
+// combining update of one tile: out = base + dt/da * R, acc_out = acc_in +
+// dt/db * R, then the 'cont' ghosts of out
 template <int NDIM, int CK>
 __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int out, double da,
                                                 int acc_in, int acc_out, double db) {
@@ -199,6 +226,7 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
     if (tile_cell<NDIM, CK>(a, tl, g, j, c)) {
       double r[2 * NDIM];
       final_cell<NDIM, NDIR, double, double>(f, g, c, r);
+      cont_ghosts_of<NDIM>(g, (double*)f.out, c);
 #pragma unroll
       for (int k = 0; k < 2 * NDIM; k++) redv[k] = fmax(redv[k], r[k]);
     }
@@ -498,7 +526,6 @@ void enqueue_step_ck(Batch& b) {
       hipLaunchKernelGGL((kb_axis<NDIM, CK, NDIM - 1>), gt, bt, 0, st, a, s.wct, s.out);
     hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
                        s.ao, s.db);
-    enqueue_ghost(b, a.reg[s.out], NE);
   }
   hipLaunchKernelGGL(kb_tick, gn, bn, 0, st, a);
 }
