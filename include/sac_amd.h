@@ -217,8 +217,8 @@ int sac_closure_enable(sac_handle h, int on, double sensor);
  * nw=13, 'cont' boundaries, 'fourstep', no gravity).  No reference
  * counterpart as one call: each tube is one serial run of the reference's
  * main loop (src/vac.t:55-106) with its own dt (getdt_courant +
- * getdt_visc), t and it, stopping at the first t >= tmax (or it >= itmax).  The ensemble advances on the device in
- * hipGraph replays of several steps.  A tube stops at t >= tmax or it >= itmax
+ * getdt_visc), t and it.  The ensemble advances on the device in hipGraph
+ * replays of several steps.  A tube stops at t >= tmax or it >= itmax
  * (state 1), at dt < dtmin = 1e-99 (state 3, the reference's third finish
  * condition, reached when a tube blows up), or when dt is not a number
  * (state 2).
@@ -251,12 +251,14 @@ int sac_batch_tune(sac_batch_handle h, int trial_steps, int *ck, int *tb, double
  * y in cells (tubes: tbx = threads*ck, tby = 1; patches: ck = 1). */
 int sac_batch_shape(sac_batch_handle h, int *ck, int *tbx, int *tby);
 /* Advance every tube to its end, replaying a graph of steps_per_graph steps
- * at most max_steps steps in total; returns the steps issued. */
+ * at most max_steps steps in total; returns the steps issued.  Once at most
+ * half of the tubes the tiles cover still run, the tiles and the graph are
+ * rebuilt over the running ones (the results do not change). */
 int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,
                   long long *steps_done);
 /* Per tube: t, it and state (0 running, 1 finished, 2 failed, 3 dt < dtmin). */
 int sac_batch_status(sac_batch_handle h, double *t, int *it, int *state);
-/* The ensemble state in the layout of sac_batch_init's w. */
+/* The ensemble state in the layout of sac_batch_init(_nd)'s w. */
 int sac_batch_download(sac_batch_handle h, double *w);
 int sac_batch_memory(sac_batch_handle h, long long *bytes);
 const char *sac_batch_last_error(void);
