@@ -168,7 +168,11 @@ def main():
                      "traffic": traffic,
                      "kernel": "RK stage = 3 x k_line<3,3,K> + k_final<3,3> + k_ghost",
                      "avg_ms": round(per_stage_ms, 4),
-                     "launches": f_n},
+                     "launches": f_n,
+                     # the measured HBM traffic (rocprof, copy-calibrated) over the
+                     # same time: how close the kernels run to the memory system
+                     "traffic_rate_GBs": (round(traffic / 1e9 / (per_stage_ms / 1e3), 1)
+                                          if traffic else None)},
         "stage_breakdown_ms": {"k_axis": round(a_ms / max(a_n, 1), 4),
                                "k_axis_per_axis": [round(tim[k][0] / max(tim[k][1], 1), 4)
                                                    for k in ("K_AXIS1", "K_AXIS2", "K_AXIS3")],
