@@ -242,11 +242,14 @@ int sac_batch_init_nd(int ndim, int ntubes, const int *ncell, int nw, const doub
                       double courantpar, double tmax, int itmax, sac_batch_handle *out);
 int sac_batch_free(sac_batch_handle h);
 /* Autotuner: times trial_steps steps for every launch shape from the
- * current state, restores the state, keeps the fastest shape.  Tubes: cells
- * per thread ck in {1,2,4} x threads per block tb in {64,128,256}.
+ * current state, restores the state, keeps the fastest shape.  Tubes: LDS
+ * line tiles of tb in {64,128,256} cells, one per thread (ck = 1).
  * Patches: one cell per thread, tiles of tbx x tb/tbx cells for tb in
  * {64,128,256} and tbx in {16,...,tb} (12 shapes). */
 int sac_batch_tune(sac_batch_handle h, int trial_steps, int *ck, int *tb, double *ms_per_step);
+/* Use launch shape `index` of the autotuner's list (tubes 0..2, patches
+ * 0..11) without timing. */
+int sac_batch_set_shape(sac_batch_handle h, int index);
 /* The current tile: cells per thread (ck) and the tile's extent along x and
  * y in cells (tubes: tbx = threads*ck, tby = 1; patches: ck = 1). */
 int sac_batch_shape(sac_batch_handle h, int *ck, int *tbx, int *tby);

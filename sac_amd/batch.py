@@ -14,6 +14,7 @@ from .device import SacError, _dp, _ip, grid_dx, load_library
 
 EXPORTS = [
     "sac_batch_init", "sac_batch_init_nd", "sac_batch_free", "sac_batch_tune", "sac_batch_shape",
+    "sac_batch_set_shape",
     "sac_batch_run", "sac_batch_status", "sac_batch_download", "sac_batch_memory",
     "sac_batch_last_error",
 ]
@@ -35,6 +36,7 @@ def _lib():
             "sac_batch_init": (I, [I, ip, I, dp, dp, dp, I, I, D, D, I, ctypes.POINTER(V)]),
             "sac_batch_init_nd": (I, [I, I, ip, I, dp, dp, dp, I, I, D, D, I, ctypes.POINTER(V)]),
             "sac_batch_shape": (I, [V, ip, ip, ip]),
+            "sac_batch_set_shape": (I, [V, I]),
             "sac_batch_free": (I, [V]),
             "sac_batch_tune": (I, [V, I, ip, ip, dp]),
             "sac_batch_run": (I, [V, I, ctypes.c_longlong, llp]),
@@ -116,6 +118,10 @@ class Ensemble:
         _check(_lib().sac_batch_shape(self._h, ctypes.byref(ck), ctypes.byref(tx),
                                       ctypes.byref(ty)))
         return ck.value, tx.value, ty.value
+
+    def set_shape(self, index):
+        """Use launch shape `index` of the autotuner's list without timing."""
+        _check(_lib().sac_batch_set_shape(self._h, index))
 
     def run(self, steps_per_graph=32, max_steps=10 ** 7):
         n = ctypes.c_longlong()

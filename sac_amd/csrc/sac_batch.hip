@@ -164,6 +164,26 @@ __global__ void __launch_bounds__(256) kb_axis(Args a, int wct, int out) {
   }
 }
 
+// Tubes: the x work as LDS line tiles of LT cells (line1_tile, sac_device.h).
+// The x kernel's first tile of a tube also clears the tube's reductions of
+// register `out`, which the stage's kb_final then fills (out != wct).
+template <int LT>
+__global__ void __launch_bounds__(LT) kb_line1(Args a, int wct, int out) {
+  __shared__ double L[line1::LV<NDIR> * (LT + 6)];
+  const Tile tl = a.tiles[blockIdx.x];
+  if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
+  if (threadIdx.x < kRed && tl.c0 == 2) red_of(a, tl.tube, out)[threadIdx.x] = 0ull;
+  const GridDesc g = tube_grid<1>(a, tl.tube);
+  const long long o = a.start[tl.tube];
+  double numax = 0.0;
+  line1_tile<LT, NDIR>(g, a.reg[wct] + o, a.bg + o, a.dx + a.dxo[tl.tube], a.R + o, a.gamma,
+                 a.visc_on, red_get(red_of(a, tl.tube, wct), 0), tl.c0, L, numax);
+  if (a.visc_on) {
+    double r[1] = {numax};
+    block_max_atomic<1>(r, a.visc + tl.tube);
+  }
+}
+
 // 'cont' ghost cells (getboundary, src/vacgrid.t:366-370) whose value is
 // that of mesh cell c of register f, written by the thread that stored c.
 // The reference fills the x regions over the whole grid along y, then the
@@ -388,14 +408,14 @@ int bfail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return bfail(SAC_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-// launch shapes the autotuner chooses from.  Tubes: cells per thread (ck)
-// and threads per block (tb), a tile being tb*ck consecutive cells of one
-// tube.  Patches: one cell per thread, tiles of tbx x (tb/tbx) cells.
+// launch shapes the autotuner chooses from.  Tubes: tiles of tb consecutive
+// cells of one tube, one cell per thread, the x work as LDS line tiles
+// (kb_line1<tb>).  Patches: one cell per thread, tiles of tbx x (tb/tbx)
+// cells.  Every shape gives the same result bit for bit.
 struct Shape {
   int ck, tb, tbx;
 };
-const Shape kShapes1[] = {{1, 64, 0},  {1, 128, 0}, {1, 256, 0}, {2, 64, 0},  {2, 128, 0},
-                          {2, 256, 0}, {4, 64, 0},  {4, 128, 0}, {4, 256, 0}};
+const Shape kShapes1[] = {{1, 64, 0}, {1, 128, 0}, {1, 256, 0}};
 const Shape kShapes2[] = {{1, 64, 16},  {1, 64, 32},  {1, 64, 64},   {1, 128, 16},
                           {1, 128, 32}, {1, 128, 64}, {1, 128, 128}, {1, 256, 16},
                           {1, 256, 32}, {1, 256, 64}, {1, 256, 128}, {1, 256, 256}};
@@ -521,7 +541,13 @@ void enqueue_step_ck(Batch& b) {
                         {S2, W, S1, 1.0, A, A, 3.0},
                         {S1, A, W, 6.0, -1, -1, 0.0}};
   for (const St& s : stages) {
-    hipLaunchKernelGGL((kb_axis<NDIM, CK, 0>), gt, bt, 0, st, a, s.wct, s.out);
+    if constexpr (NDIM == 1) {
+      if (b.shape.tb == 64) hipLaunchKernelGGL(kb_line1<64>, gt, bt, 0, st, a, s.wct, s.out);
+      else if (b.shape.tb == 128) hipLaunchKernelGGL(kb_line1<128>, gt, bt, 0, st, a, s.wct, s.out);
+      else hipLaunchKernelGGL(kb_line1<256>, gt, bt, 0, st, a, s.wct, s.out);
+    } else {
+      hipLaunchKernelGGL((kb_axis<NDIM, CK, 0>), gt, bt, 0, st, a, s.wct, s.out);
+    }
     if (NDIM == 2)
       hipLaunchKernelGGL((kb_axis<NDIM, CK, NDIM - 1>), gt, bt, 0, st, a, s.wct, s.out);
     hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
@@ -531,30 +557,20 @@ void enqueue_step_ck(Batch& b) {
 }
 
 void enqueue_step(Batch& b) {
-  if (b.ndim == 2) {
+  if (b.ndim == 2)
     enqueue_step_ck<2, 1>(b);
-    return;
-  }
-  switch (b.shape.ck) {
-    case 1: enqueue_step_ck<1, 1>(b); break;
-    case 2: enqueue_step_ck<1, 2>(b); break;
-    default: enqueue_step_ck<1, 4>(b); break;
-  }
+  else
+    enqueue_step_ck<1, 1>(b);
 }
 
 void enqueue_cmax(Batch& b, int slot) {
   const dim3 gn((b.ntubes + 255) / 256), bn(256);
   hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, b.stream, b.a, slot);
   const dim3 gt(b.ntiles), bt(b.shape.tb);
-  if (b.ndim == 2) {
+  if (b.ndim == 2)
     hipLaunchKernelGGL((kb_cmax<2, 1>), gt, bt, 0, b.stream, b.a, slot);
-    return;
-  }
-  switch (b.shape.ck) {
-    case 1: hipLaunchKernelGGL((kb_cmax<1, 1>), gt, bt, 0, b.stream, b.a, slot); break;
-    case 2: hipLaunchKernelGGL((kb_cmax<1, 2>), gt, bt, 0, b.stream, b.a, slot); break;
-    default: hipLaunchKernelGGL((kb_cmax<1, 4>), gt, bt, 0, b.stream, b.a, slot); break;
-  }
+  else
+    hipLaunchKernelGGL((kb_cmax<1, 1>), gt, bt, 0, b.stream, b.a, slot);
 }
 
 // capture `steps` time steps (plus the running-tube counter reset) in one graph
@@ -768,7 +784,7 @@ int sac_batch_init_nd(int ndim, int ntubes, const int* ncell, int nw, const doub
   a.dtmin = 1.0e-99;
   a.itmax = itmax;
   a.running = b->d_running;
-  if ((rc = set_shape(*b, ndim == 1 ? Shape{2, 128, 0} : Shape{1, 256, 64}))) return bail(rc);
+  if ((rc = set_shape(*b, ndim == 1 ? Shape{1, 256, 0} : Shape{1, 256, 64}))) return bail(rc);
   // initial getboundary (background and state) and cmax of W
   enqueue_ghost(*b, b->d_bg, NBG);
   enqueue_ghost(*b, b->d_reg[SAC_SLOT_W], NE);
@@ -870,6 +886,17 @@ int sac_batch_tune(sac_batch_handle h, int trial_steps, int* ck, int* tb, double
   if (tb) *tb = bs.tb;
   if (ms_per_step) *ms_per_step = best;
   return 0;
+}
+
+int sac_batch_set_shape(sac_batch_handle h, int index) {
+  if (!h) return bfail(SAC_E_ARG, "null handle");
+  Batch& b = *(Batch*)h;
+  const Shape* shapes = (b.ndim == 1) ? kShapes1 : kShapes2;
+  const int nshape = (b.ndim == 1) ? (int)(sizeof(kShapes1) / sizeof(Shape))
+                                   : (int)(sizeof(kShapes2) / sizeof(Shape));
+  if (index < 0 || index >= nshape)
+    return bfail(SAC_E_ARG, "shape index %d outside 0..%d", index, nshape - 1);
+  return set_shape(b, shapes[index]);
 }
 
 int sac_batch_shape(sac_batch_handle h, int* ck, int* tbx, int* tby) {

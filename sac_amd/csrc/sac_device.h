@@ -539,6 +539,132 @@ __device__ __forceinline__ void axis_group(const AxisArgs& a, const GridDesc& g,
 }
 
 // ---------------------------------------------------------------------
+// 1-D grids in FP64: the x work of a tile of LT consecutive cells c0..c0+LT-1
+// with LDS staging.  The LT+6 points the tile reads (cells -3..+3 around,
+// the mirror layer beyond the grid as in axis_group) are loaded and
+// evaluated once each: flux and velocity along x (getflux + transport),
+// p_Tb and bg_x of the background energy terms, rho_t and m_x of the face
+// averages, and setnu's q for rho', the thermal energy and v_x.  Each
+// thread then assembles its cell from its neighbours' values with the
+// expressions of axis_group<1, NDIR, 0, 1> in the same order, but with each
+// stencil point evaluated once instead of up to seven times.  R is stored
+// for the mesh cells of the tile; numax gets the thread's largest nu.
+namespace line1 {
+// values per point: F[NE], v, pTb, bg, rho_t, m, q[3]
+template <int NDIR>
+constexpr int LV = 2 + 2 * NDIR + 8;
+}  // namespace line1
+
+template <int LT, int NDIR>
+__device__ __forceinline__ void line1_tile(const GridDesc& g, const double* __restrict__ W,
+                                           const double* __restrict__ B,
+                                           const double* __restrict__ dx,
+                                           double* __restrict__ Rout, double gamma, int visc_on,
+                                           double ctk, int c0, double* __restrict__ L,
+                                           double& numax) {
+  constexpr int NE = 2 + 2 * NDIR, LP = LT + 6;
+  const long long fs = g.fs;
+  const int GK = g.G[0], MloK = g.Mlo[0], MhiK = g.Mhi[0];
+  const int t = (int)threadIdx.x;
+  auto at = [&](int v, int i) -> double& { return L[v * LP + i]; };
+  for (int i = t; i < LP; i += LT) {
+    int pK = c0 - 3 + i;
+    if (pK < 0) pK = MloK + 3;       // mirror layer below the grid
+    if (pK > GK - 1) pK = MhiK - 3;  // and above it
+    Pt<NDIR, double> p;
+    load_pt<NDIR, double, double>(W, B, fs, pK, p);
+    double F[NE], v[1];
+    flux_pt<NDIR, 1, double>(p, 0, gamma, F, v);
+#pragma unroll
+    for (int n = 0; n < NE; n++) at(n, i) = F[n];
+    at(NE, i) = v[0];
+    at(NE + 1, i) = ptotal_bg<NDIR, double>(p, gamma);
+    at(NE + 2, i) = p.bg[0];
+    at(NE + 3, i) = p.rho + p.rhob;
+    at(NE + 4, i) = p.m[0];
+    {
+#pragma clang fp contract(off)
+      const double inv = recip<double>(p.rho + p.rhob);
+      double sb = 0.0, sm = 0.0;
+#pragma unroll
+      for (int cd = 0; cd < NDIR; cd++) {
+        sb += p.b[cd] * p.b[cd];
+        sm += p.m[cd] * p.m[cd];
+      }
+      at(NE + 5, i) = p.rho;
+      at(NE + 6, i) = p.e - 0.5 * (sb + sm * inv);
+      at(NE + 7, i) = p.m[0] * inv;
+    }
+  }
+  __syncthreads();
+  const int c = c0 + t;
+  const bool valid = c <= MhiK;
+  const int k = t + 3;  // LDS index of the cell
+  const bool fluxc = valid && c >= MloK + 2 && c <= MhiK - 2;
+  const bool z1 = (c == MloK && !g.per_lo[0]) || (c == MhiK && !g.per_hi[0]);
+  const double dxd = dx[valid ? c : MhiK];
+  const double dxk = dxd, idx = 1.0 / dxd;
+  double R[NE];
+  {
+    const double pTb0 = at(NE + 1, k), bgk = at(NE + 2, k), bgl = bgk;
+    const double inv12 = idx / 12.0;
+    double D1[NE], E1[NE];
+#pragma unroll
+    for (int n = 0; n < NE; n++) D1[n] = two_diff(at(n, k + 1), at(n, k - 1), E1[n]);
+    const double dv1 = at(NE, k + 1) - at(NE, k - 1);
+#pragma unroll
+    for (int n = 0; n < NE; n++) {
+      double e2;
+      const double D2 = two_diff(at(n, k + 2), at(n, k - 2), e2);
+      R[n] = ((D2 - 8.0 * D1[n]) + (e2 - 8.0 * E1[n])) * inv12;
+    }
+    double re = 0;
+    const double Dv = -((at(NE, k + 2) - at(NE, k - 2)) - 8.0 * dv1) * inv12;
+    re -= Dv * pTb0;
+    re += bgk * bgl * Dv;
+    R[1 + NDIR] += re;
+#pragma unroll
+    for (int n = 0; n < NE; n++) R[n] = fluxc ? R[n] : 0.0;
+  }
+  if (visc_on) {
+    const double rtw0 = at(NE + 3, k - 1), rtw1 = at(NE + 3, k), rtw2 = at(NE + 3, k + 1);
+#pragma unroll
+    for (int pq = 0; pq < 3; pq++) {
+#pragma clang fp contract(off)
+      double q[7];
+#pragma unroll
+      for (int s2 = 0; s2 < 7; s2++) q[s2] = at(NE + 5 + pq, k - 3 + s2);
+      const double ch = (pq == 2) ? 0.4 : 0.04;
+      const double Kc = ctk * ch;
+      const double ratio0 = face_ratio<double>(q), ratio1 = face_ratio<double>(q + 1);
+      const double nuR = Kc * ratio1 * dxk;
+      const double nuL = Kc * ratio0 * dxk;
+      numax = valid ? fmax(numax, fmax(nuR, nuL)) : numax;
+      const double qm = q[2], q0 = q[3], qp = q[4];
+      const double gL = z1 ? 0.0 : (q0 - qm) * idx;
+      const double gR = z1 ? 0.0 : (qp - q0) * idx;
+      const double tL = nuL * gL, tR = nuR * gR;
+      if (pq == 0) {
+        R[0] += (tR - tL) * idx;
+      } else if (pq == 1) {
+        R[1 + NDIR] += (tR - tL) * idx;
+      } else {
+        const double rhoL = (rtw1 + rtw0) * 0.5, rhoR = (rtw2 + rtw1) * 0.5;
+        const double m0 = at(NE + 4, k);
+        const double mL = (m0 + at(NE + 4, k - 1)) * 0.5;
+        const double mR = (at(NE + 4, k + 1) + m0) * 0.5;
+        R[1] += (rhoR * tR - rhoL * tL) * idx;
+        R[1 + NDIR] += (mR * tR - mL * tL) * idx;
+      }
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int n = 0; n < NE; n++) Rout[n * fs + c] = R[n];
+  }
+}
+
+// ---------------------------------------------------------------------
 // kernel 2: gravity, derivatives of the cross-term factors, the register
 // update out = base + qa*R (acc_out = acc_in + qb*R) and the cmax
 // reductions of the result.

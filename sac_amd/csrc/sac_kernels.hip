@@ -82,6 +82,20 @@ __global__ void __launch_bounds__((line::Tile<K, C>::T)) k_line(AxisArgs a, int 
   }
 }
 
+// kernel 1, 1-D grids in FP64: LDS line tiles of 256 cells (line1_tile)
+template <int NDIR>
+__global__ void __launch_bounds__(256) k_line1(AxisArgs a) {
+  __shared__ double L[line1::LV<NDIR> * (256 + 6)];
+  double numax = 0.0;
+  line1_tile<256, NDIR>(a.g, (const double*)a.wct, a.bg, a.dx[0], (double*)a.R, a.gamma, a.visc_on,
+                  a.visc_on ? red_get(a.ctot_src, 0) : 0.0, a.g.Mlo[0] + 256 * (int)blockIdx.x, L,
+                  numax);
+  if (a.visc_on) {
+    double r2[1] = {numax};
+    block_max_atomic<1>(r2, a.red_visc);
+  }
+}
+
 // segment length along K: whole lines unless that leaves fewer than 2048
 // blocks (then halves, down to 32 cells)
 template <int NDIM, int K, typename C>
@@ -281,6 +295,11 @@ static void launch_nd(int which, int axis, const void* args, hipStream_t st, lon
               hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2, S, C, true>), dim3(nb), dim3(bs), 0, st, A);
         }
       }
+      return;
+    }
+    if constexpr (NDIM == 1 && sizeof(S) == sizeof(double) && SAC_LINE_TILES) {
+      const int n = A.g.Mhi[0] - A.g.Mlo[0] + 1;
+      hipLaunchKernelGGL(k_line1<NDIR>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A);
       return;
     }
     if constexpr (NDIM > 1 && SAC_LINE_TILES) {

@@ -264,3 +264,30 @@ def test_tuning_in_the_middle_of_a_run_keeps_the_run_state(mods):
     np.testing.assert_array_equal(se, sr)
     for x, y in zip(we, wr):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("ndim,nshape", [(1, 3), (2, 12)])
+def test_every_launch_shape_gives_the_same_run(mods, ndim, nshape):
+    """Each shape of the autotuner's list (tubes: line tiles of 64, 128 and
+    256 cells; patches: 12 tiles) gives the run of the default shape bit for
+    bit."""
+    batch, _ = mods
+    probs = (_small_ensemble(batch, count=6, seed=29) if ndim == 1
+             else _small_patches(batch, count=4, seed=31))
+    tmax = 0.01 if ndim == 1 else 0.004
+    ref = batch.Ensemble(probs, tmax=tmax)
+    ref.run(steps_per_graph=8)
+    wr = ref.download()
+    tr, ir, _ = ref.status()
+    ref.close()
+    for k in range(nshape):
+        e = batch.Ensemble(probs, tmax=tmax)
+        e.set_shape(k)
+        e.run(steps_per_graph=8)
+        we = e.download()
+        te, ie, _ = e.status()
+        e.close()
+        np.testing.assert_array_equal(ie, ir, err_msg=f"shape {k}")
+        np.testing.assert_array_equal(te, tr, err_msg=f"shape {k}")
+        for i, (x, y) in enumerate(zip(we, wr)):
+            np.testing.assert_array_equal(x, y, err_msg=f"shape {k}, tube {i}")
