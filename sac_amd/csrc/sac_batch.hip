@@ -164,26 +164,6 @@ __global__ void __launch_bounds__(256) kb_axis(Args a, int wct, int out) {
   }
 }
 
-// Tubes: the x work as LDS line tiles of LT cells (line1_tile, sac_device.h).
-// The x kernel's first tile of a tube also clears the tube's reductions of
-// register `out`, which the stage's kb_final then fills (out != wct).
-template <int LT>
-__global__ void __launch_bounds__(LT) kb_line1(Args a, int wct, int out) {
-  __shared__ double L[line1::LV<NDIR> * (LT + 6)];
-  const Tile tl = a.tiles[blockIdx.x];
-  if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
-  if (threadIdx.x < kRed && tl.c0 == 2) red_of(a, tl.tube, out)[threadIdx.x] = 0ull;
-  const GridDesc g = tube_grid<1>(a, tl.tube);
-  const long long o = a.start[tl.tube];
-  double numax = 0.0;
-  line1_tile<LT, NDIR>(g, a.reg[wct] + o, a.bg + o, a.dx + a.dxo[tl.tube], a.R + o, a.gamma,
-                 a.visc_on, red_get(red_of(a, tl.tube, wct), 0), tl.c0, L, numax);
-  if (a.visc_on) {
-    double r[1] = {numax};
-    block_max_atomic<1>(r, a.visc + tl.tube);
-  }
-}
-
 // 'cont' ghost cells (getboundary, src/vacgrid.t:366-370) whose value is
 // that of mesh cell c of register f, written by the thread that stored c.
 // The reference fills the x regions over the whole grid along y, then the
@@ -252,6 +232,53 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
     }
   }
   block_max_atomic<2 * NDIM>(redv, red_of(a, tl.tube, out));
+}
+
+// Tubes: one whole RK stage of a tile of LT cells in one kernel: the x work
+// as an LDS line tile (line1_tile, sac_device.h, also the x kernel of 1-D
+// FP64 grids), then the combining update of the tile's cells (final_cell;
+// the thread that stored R of its cell reads it back, so no other block is
+// waited for) with the 'cont' ghosts.  The reductions of `out` must have
+// been cleared by kb_zero_red.
+template <int LT>
+__global__ void __launch_bounds__(LT) kb_tube_stage(Args a, int wct, int base, int out, double da,
+                                                   int acc_in, int acc_out, double db) {
+  __shared__ double L[line1::LV<NDIR> * (LT + 6)];
+  const Tile tl = a.tiles[blockIdx.x];
+  if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
+  const GridDesc g = tube_grid<1>(a, tl.tube);
+  const long long o = a.start[tl.tube];
+  double numax = 0.0;
+  line1_tile<LT, NDIR>(g, a.reg[wct] + o, a.bg + o, a.dx + a.dxo[tl.tube], a.R + o, a.gamma,
+                       a.visc_on, red_get(red_of(a, tl.tube, wct), 0), tl.c0, L, numax);
+  __syncthreads();  // orders each thread's R stores before its loads below
+  const double dt = a.dt[tl.tube];
+  FinalArgs f{};
+  f.g = g;
+  f.prec = PREC_FP64;
+  f.wct = a.reg[wct] + o;
+  f.bg = a.bg + o;
+  f.dx[0] = a.dx + a.dxo[tl.tube];
+  f.R = a.R + o;
+  f.base = a.reg[base] + o;
+  f.out = a.reg[out] + o;
+  f.acc_in = acc_in >= 0 ? a.reg[acc_in] + o : nullptr;
+  f.acc_out = acc_out >= 0 ? a.reg[acc_out] + o : nullptr;
+  f.qa = dt / da;
+  f.qb = acc_out >= 0 ? dt / db : 0.0;
+  f.gamma = a.gamma;
+  f.visc_on = a.visc_on;
+  double redv[2] = {0.0, 0.0};
+  int c[3] = {tl.c0 + (int)threadIdx.x, 0, 0};
+  if (c[0] <= g.Mhi[0]) {
+    final_cell<1, NDIR, double, double>(f, g, c, redv);
+    cont_ghosts_of<1>(g, (double*)f.out, c);
+  }
+  if (a.visc_on) {
+    double r[1] = {numax};
+    block_max_atomic<1>(r, a.visc + tl.tube);
+  }
+  block_max_atomic<2>(redv, red_of(a, tl.tube, out));
 }
 
 // cmax reductions of register `slot` of every running tube
@@ -409,8 +436,7 @@ int bfail(int code, const char* fmt, ...) {
   } while (0)
 
 // launch shapes the autotuner chooses from.  Tubes: tiles of tb consecutive
-// cells of one tube, one cell per thread, the x work as LDS line tiles
-// (kb_line1<tb>).  Patches: one cell per thread, tiles of tbx x (tb/tbx)
+// cells of one tube, one cell per thread (kb_tube_stage<tb>).  Patches: one cell per thread, tiles of tbx x (tb/tbx)
 // cells.  Every shape gives the same result bit for bit.
 struct Shape {
   int ck, tb, tbx;
@@ -542,9 +568,17 @@ void enqueue_step_ck(Batch& b) {
                         {S1, A, W, 6.0, -1, -1, 0.0}};
   for (const St& s : stages) {
     if constexpr (NDIM == 1) {
-      if (b.shape.tb == 64) hipLaunchKernelGGL(kb_line1<64>, gt, bt, 0, st, a, s.wct, s.out);
-      else if (b.shape.tb == 128) hipLaunchKernelGGL(kb_line1<128>, gt, bt, 0, st, a, s.wct, s.out);
-      else hipLaunchKernelGGL(kb_line1<256>, gt, bt, 0, st, a, s.wct, s.out);
+      hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, st, a, s.out);
+      if (b.shape.tb == 64)
+        hipLaunchKernelGGL(kb_tube_stage<64>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
+                           s.ao, s.db);
+      else if (b.shape.tb == 128)
+        hipLaunchKernelGGL(kb_tube_stage<128>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
+                           s.ao, s.db);
+      else
+        hipLaunchKernelGGL(kb_tube_stage<256>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
+                           s.ao, s.db);
+      continue;
     } else {
       hipLaunchKernelGGL((kb_axis<NDIM, CK, 0>), gt, bt, 0, st, a, s.wct, s.out);
     }
