@@ -15,11 +15,10 @@
 // tmax; finished tubes skip their tiles.
 //
 // The line work and the combining update are the grid path's own device
-// functions (axis_group, final_cell) applied to a per-tube grid descriptor,
-// so a 1-D tube advanced here follows the same arithmetic as the same tube
-// run alone through sac_dev_*.  A 2-D patch adds the per-axis kernels of
-// both axes and the cross-term factors T, as the grid path's line-group
-// kernels (k_axis) do.
+// functions (line1_tile for tubes, line_tile for patches, final_cell)
+// applied to a per-tube grid descriptor, so a tube or patch advanced here
+// follows the same arithmetic as the same tube or patch run alone through
+// sac_dev_*, bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +32,7 @@
 
 #include "../../include/sac_amd.h"
 #include "sac_device.h"
+#include "sac_line.h"
 
 namespace sac {
 namespace batch {
@@ -68,6 +68,8 @@ struct Args {
   int tbx;                   // tile width along x in cells (2-D)
   const int* active;         // tubes still running when the tiles were built
   int nactive;
+  const Tile* ltiles[2];     // patches: (patch, logical block) of the line tiles of x and y
+  int nltiles[2];
   double* reg[kSlots];
   const double* bg;
   const double* dx;          // cell widths per tube: G_x values, then G_y values
@@ -130,16 +132,19 @@ __device__ __forceinline__ bool tile_cell(const Args& a, const Tile& tl, const G
   return c[0] <= g.Mhi[0] && c[1] <= g.Mhi[1];
 }
 
-// line work of axis K for one tile: each thread one group of CK cells.  The
-// x kernel's first tile of a tube also clears the tube's reductions of
-// register `out`, which the stage's kb_final then fills (out != wct).
-template <int NDIM, int CK, int K>
-__global__ void __launch_bounds__(256) kb_axis(Args a, int wct, int out) {
-  const Tile tl = a.tiles[blockIdx.x];
+// Patches: the work of axis K as the grid path's LDS line tiles (line_tile,
+// sac_line.h) on the patch's own grid; ltiles[K] holds (patch, logical
+// block).  The y kernel adds the y derivatives of the x kernel's cross
+// factors, as on a grid.  The x kernel's first block of a patch also clears
+// the patch's reductions of register `out`.
+template <int K>
+__global__ void __launch_bounds__((line::Tile<K, double>::T)) kb_pline(Args a, int wct, int out) {
+  __shared__ double L[line::Slots<2, NDIR>::NV * line::Tile<K, double>::RR *
+                      line::Tile<K, double>::W];
+  const Tile tl = a.ltiles[K][blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
-  if (K == 0 && threadIdx.x < kRed && tl.c0 == 2 && (NDIM == 1 || tl.c1 == 2))
-    red_of(a, tl.tube, out)[threadIdx.x] = 0ull;
-  const GridDesc g = tube_grid<NDIM>(a, tl.tube);
+  if (K == 0 && threadIdx.x < kRed && tl.c0 == 0) red_of(a, tl.tube, out)[threadIdx.x] = 0ull;
+  const GridDesc g = tube_grid<2>(a, tl.tube);
   const long long o = a.start[tl.tube];
   AxisArgs x{};
   x.g = g;
@@ -147,17 +152,17 @@ __global__ void __launch_bounds__(256) kb_axis(Args a, int wct, int out) {
   x.wct = a.reg[wct] + o;
   x.bg = a.bg + o;
   x.dx[0] = a.dx + a.dxo[tl.tube];
-  if (NDIM == 2) x.dx[1] = x.dx[0] + g.G[0];
+  x.dx[1] = x.dx[0] + g.G[0];
   x.R = a.R + o;
-  x.T = (NDIM == 2) ? a.T + o : nullptr;
+  x.T = a.T + o;
   x.gamma = a.gamma;
   x.visc_on = a.visc_on;
   x.ctot_src = red_of(a, tl.tube, wct);
   x.red_visc = a.visc + tl.tube;
+  x.xcd = 1;
+  x.cross_last = (K == 1) ? 1 : 0;
   double numax = 0.0;
-  int c[3];
-  if (tile_cell<NDIM, CK>(a, tl, g, 0, c))
-    axis_group<NDIM, NDIR, K, CK, double, double>(x, g, c, numax);
+  line::line_tile<2, NDIR, K, double, double>(x, g.Mhi[K] - g.Mlo[K] + 1, L, numax, tl.c0);
   if (a.visc_on) {
     double r[1] = {numax};
     block_max_atomic<1>(r, x.red_visc);
@@ -217,6 +222,7 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
   f.qb = acc_out >= 0 ? dt / db : 0.0;
   f.gamma = a.gamma;
   f.visc_on = a.visc_on;
+  f.skip_last = (NDIM == 2) ? 1 : 0;  // the y line tiles added their cross terms
   double redv[2 * NDIM];
 #pragma unroll
   for (int k = 0; k < 2 * NDIM; k++) redv[k] = 0.0;
@@ -465,6 +471,7 @@ struct Batch {
   int* d_pitch = nullptr;
   long long* d_dxo = nullptr;
   Tile* d_tiles = nullptr;
+  Tile* d_ltiles[2] = {nullptr, nullptr};
   int* d_active = nullptr;
   double* d_reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   double* d_bg = nullptr;
@@ -514,6 +521,34 @@ int set_shape(Batch& b, Shape s) {
         for (int c0 = 0; c0 < b.nxy[2 * i]; c0 += s.tbx) tiles.push_back({i, 2 + c0, 2 + c1});
   }
   if (tiles.empty()) tiles.push_back({b.active.empty() ? 0 : b.active[0], 2, 2});
+  if (b.ndim == 2) {
+    // line tiles of each axis: every logical block of each running patch's grid
+    for (int K = 0; K < 2; K++) {
+      std::vector<Tile> lt;
+      for (int i : b.active) {
+        sac::GridDesc g{};
+        g.ndim = 2;
+        g.ndir = NDIR;
+        g.G[0] = b.gx(i);
+        g.G[1] = b.gy(i);
+        g.G[2] = 1;
+        g.Mlo[0] = g.Mlo[1] = 2;
+        g.Mhi[0] = g.G[0] - 3;
+        g.Mhi[1] = g.G[1] - 3;
+        const int seg = g.Mhi[K] - g.Mlo[K] + 1;
+        const long long nb = (K == 0) ? sac::line::nblocks<2, 0, double>(g, seg)
+                                      : sac::line::nblocks<2, 1, double>(g, seg);
+        for (long long k = 0; k < nb; k++) lt.push_back({i, (int)k, 0});
+      }
+      if (lt.empty()) lt.push_back({b.active.empty() ? 0 : b.active[0], 1 << 30, 0});
+      if (b.d_ltiles[K]) hipFree(b.d_ltiles[K]);
+      b.d_ltiles[K] = nullptr;
+      BCHK(hipMalloc((void**)&b.d_ltiles[K], sizeof(Tile) * lt.size()));
+      BCHK(hipMemcpy(b.d_ltiles[K], lt.data(), sizeof(Tile) * lt.size(), hipMemcpyHostToDevice));
+      b.a.ltiles[K] = b.d_ltiles[K];
+      b.a.nltiles[K] = (int)lt.size();
+    }
+  }
   if (b.d_tiles) hipFree(b.d_tiles);
   b.d_tiles = nullptr;
   BCHK(hipMalloc((void**)&b.d_tiles, sizeof(Tile) * tiles.size()));
@@ -580,10 +615,11 @@ void enqueue_step_ck(Batch& b) {
                            s.ao, s.db);
       continue;
     } else {
-      hipLaunchKernelGGL((kb_axis<NDIM, CK, 0>), gt, bt, 0, st, a, s.wct, s.out);
+      hipLaunchKernelGGL(kb_pline<0>, dim3(a.nltiles[0]), dim3(sac::line::Tile<0, double>::T), 0, st,
+                         a, s.wct, s.out);
+      hipLaunchKernelGGL(kb_pline<1>, dim3(a.nltiles[1]), dim3(sac::line::Tile<1, double>::T), 0, st,
+                         a, s.wct, s.out);
     }
-    if (NDIM == 2)
-      hipLaunchKernelGGL((kb_axis<NDIM, CK, NDIM - 1>), gt, bt, 0, st, a, s.wct, s.out);
     hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
                        s.ao, s.db);
   }
@@ -852,6 +888,8 @@ int sac_batch_free(sac_batch_handle h) {
   hipFree(b->d_pitch);
   hipFree(b->d_dxo);
   hipFree(b->d_tiles);
+  hipFree(b->d_ltiles[0]);
+  hipFree(b->d_ltiles[1]);
   hipFree(b->d_active);
   hipFree(b->d_red);
   hipFree(b->d_visc);

@@ -104,9 +104,11 @@ __host__ __device__ inline long long nblocks(const GridDesc& g, int seg) {
 
 // LEARNED: the face ratios of setnu come from a.RF (learned closure, FP32,
 // stored at the face's left cell; < 0 where the exact ratio is required).
+// blk >= 0 gives the block's logical index directly (patch ensembles, one
+// grid per patch); otherwise it follows from blockIdx.x.
 template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
 __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restrict__ L,
-                                          C& numax) {
+                                          C& numax, long long blk = -1) {
   using SL = Slots<NDIM, NDIR>;
   constexpr int NE = SL::NE, NP = SL::NP;
   constexpr int A = Axes<NDIM, K>::A, B = Axes<NDIM, K>::B;
@@ -129,7 +131,9 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
   // and in consecutive rounds, so the line is fetched into one L2.  The
   // launch is padded to a multiple of 16 blocks; the padding blocks exit.
   long long b = blockIdx.x;
-  if constexpr (A == 0) {
+  if (blk >= 0) {
+    b = blk;
+  } else if constexpr (A == 0) {
     const long long x = b % 8, r = b / 8;
     b = 2 * (x + 8 * (r / 2)) + (r % 2);
   }

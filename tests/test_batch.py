@@ -291,3 +291,23 @@ def test_every_launch_shape_gives_the_same_run(mods, ndim, nshape):
         np.testing.assert_array_equal(te, tr, err_msg=f"shape {k}")
         for i, (x, y) in enumerate(zip(we, wr)):
             np.testing.assert_array_equal(x, y, err_msg=f"shape {k}, tube {i}")
+
+
+def test_patch_ensemble_bitwise_equals_single_patch_grid_runs(mods):
+    """Patches use the grid path's own line-tile kernels on their own grids
+    (and the same final update and ghost values), so each patch run to tmax
+    in the ensemble equals the same patch run alone through sac_dev_* bit
+    for bit, dt sequence included."""
+    batch, device = mods
+    patches = _small_patches(batch, count=5, seed=13)
+    tmax = 0.01
+    ens = batch.Ensemble(patches, tmax=tmax)
+    ens.run(steps_per_graph=4)
+    t, it, st = ens.status()
+    ws = ens.download()
+    ens.close()
+    assert (st == 1).all(), st
+    for i, prob in enumerate(patches):
+        w1, dts = device.run(prob, steps=10 ** 6, tmax=tmax)
+        assert it[i] == len(dts), (i, it[i], len(dts))
+        np.testing.assert_array_equal(ws[i][:8], w1[:8], err_msg=f"patch {i}")
