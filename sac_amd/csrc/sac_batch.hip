@@ -68,7 +68,7 @@ struct Args {
   int tbx;                   // tile width along x in cells (2-D)
   const int* active;         // tubes still running when the tiles were built
   int nactive;
-  const Tile* ltiles[2];     // patches: (patch, logical block) of the line tiles of x and y
+  const Tile* ltiles[2];     // patches: line tiles of x and y as (patch, logical block, segment)
   int nltiles[2];
   double* reg[kSlots];
   const double* bg;
@@ -134,7 +134,7 @@ __device__ __forceinline__ bool tile_cell(const Args& a, const Tile& tl, const G
 
 // Patches: the work of axis K as the grid path's LDS line tiles (line_tile,
 // sac_line.h) on the patch's own grid; ltiles[K] holds (patch, logical
-// block).  The y kernel adds the y derivatives of the x kernel's cross
+// block, segment length along K).  The y kernel adds the y derivatives of the x kernel's cross
 // factors, as on a grid.  The x kernel's first block of a patch also clears
 // the patch's reductions of register `out`.
 template <int K>
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__((line::Tile<K, double>::T)) kb_pline(Args a, i
   x.xcd = 1;
   x.cross_last = (K == 1) ? 1 : 0;
   double numax = 0.0;
-  line::line_tile<2, NDIR, K, double, double>(x, g.Mhi[K] - g.Mlo[K] + 1, L, numax, tl.c0);
+  line::line_tile<2, NDIR, K, double, double>(x, tl.c1, L, numax, tl.c0);
   if (a.visc_on) {
     double r[1] = {numax};
     block_max_atomic<1>(r, x.red_visc);
@@ -508,6 +508,9 @@ int balloc(Batch& b, void** p, size_t bytes) {
   return 0;
 }
 
+// patch counts at or below which the line tiles use short segments
+constexpr size_t kFewPatches = 64;
+
 int set_shape(Batch& b, Shape s) {
   std::vector<Tile> tiles;
   if (b.ndim == 1) {
@@ -522,7 +525,11 @@ int set_shape(Batch& b, Shape s) {
   }
   if (tiles.empty()) tiles.push_back({b.active.empty() ? 0 : b.active[0], 2, 2});
   if (b.ndim == 2) {
-    // line tiles of each axis: every logical block of each running patch's grid
+    // line tiles of each axis: every logical block of each running patch's
+    // grid.  Whole lines while many patches run; once few remain (their
+    // steps are latency-bound), segments of 32 cells give more, shorter
+    // blocks.  The result does not depend on the segment length.
+    const bool few = b.active.size() <= kFewPatches;
     for (int K = 0; K < 2; K++) {
       std::vector<Tile> lt;
       for (int i : b.active) {
@@ -535,12 +542,13 @@ int set_shape(Batch& b, Shape s) {
         g.Mlo[0] = g.Mlo[1] = 2;
         g.Mhi[0] = g.G[0] - 3;
         g.Mhi[1] = g.G[1] - 3;
-        const int seg = g.Mhi[K] - g.Mlo[K] + 1;
+        const int ext = g.Mhi[K] - g.Mlo[K] + 1;
+        const int seg = few ? std::min(ext, 32) : ext;
         const long long nb = (K == 0) ? sac::line::nblocks<2, 0, double>(g, seg)
                                       : sac::line::nblocks<2, 1, double>(g, seg);
-        for (long long k = 0; k < nb; k++) lt.push_back({i, (int)k, 0});
+        for (long long k = 0; k < nb; k++) lt.push_back({i, (int)k, seg});
       }
-      if (lt.empty()) lt.push_back({b.active.empty() ? 0 : b.active[0], 1 << 30, 0});
+      if (lt.empty()) lt.push_back({b.active.empty() ? 0 : b.active[0], 1 << 30, 1});
       if (b.d_ltiles[K]) hipFree(b.d_ltiles[K]);
       b.d_ltiles[K] = nullptr;
       BCHK(hipMalloc((void**)&b.d_ltiles[K], sizeof(Tile) * lt.size()));
