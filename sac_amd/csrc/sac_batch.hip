@@ -115,8 +115,8 @@ __device__ __forceinline__ unsigned long long* red_of(const Args& a, int tube, i
   return a.red + ((size_t)tube * kSlots + slot) * kRed;
 }
 
-// cell of thread `tid`, cell j of its group of CK along x (tubes), or the
-// tile position (patches, CK = 1); false outside the mesh
+// cell of thread `tid`: cell j of its group of CK along x (tubes), or its
+// position in the tile (patches, CK = 1); false outside the mesh
 template <int NDIM, int CK>
 __device__ __forceinline__ bool tile_cell(const Args& a, const Tile& tl, const GridDesc& g, int j,
                                           int (&c)[3]) {
@@ -134,9 +134,9 @@ __device__ __forceinline__ bool tile_cell(const Args& a, const Tile& tl, const G
 
 // Patches: the work of axis K as the grid path's LDS line tiles (line_tile,
 // sac_line.h) on the patch's own grid; ltiles[K] holds (patch, logical
-// block, segment length along K).  The y kernel adds the y derivatives of the x kernel's cross
-// factors, as on a grid.  The x kernel's first block of a patch also clears
-// the patch's reductions of register `out`.
+// block, segment length along K).  The y kernel adds the y derivatives of
+// the x kernel's cross factors, as on a grid.  The x kernel's first block
+// of a patch also clears the patch's reductions of register `out`.
 template <int K>
 __global__ void __launch_bounds__((line::Tile<K, double>::T)) kb_pline(Args a, int wct, int out) {
   __shared__ double L[line::Slots<2, NDIR>::NV * line::Tile<K, double>::RR *
@@ -195,8 +195,9 @@ __device__ __forceinline__ void cont_ghosts_of(const GridDesc& g, double* f, con
   }
 }
 
-// combining update of one tile: out = base + dt/da * R, acc_out = acc_in +
-// dt/db * R, then the 'cont' ghosts of out
+// Patches: combining update of one tile: out = base + dt/da * R, acc_out =
+// acc_in + dt/db * R, with the x derivatives of the cross factors, then the
+// 'cont' ghosts of out
 template <int NDIM, int CK>
 __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int out, double da,
                                                 int acc_in, int acc_out, double db) {
@@ -317,8 +318,9 @@ __global__ void __launch_bounds__(256) kb_cmax(Args a, int slot) {
   block_max_atomic<2 * NDIM>(redv, red_of(a, tl.tube, slot));
 }
 
-// 'cont' ghost cells of nfield packed fields f: 2 per side, copies of the
-// edge cell (getboundary, src/vacgrid.t:366-370)
+// 'cont' ghost cells of nfield packed fields f of the tubes: 2 per side,
+// copies of the edge cell (getboundary, src/vacgrid.t:366-370), for the
+// initial fill
 __global__ void kb_ghost(Args a, double* f, int nfield) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int k = (int)(i & 3);
@@ -332,8 +334,8 @@ __global__ void kb_ghost(Args a, double* f, int nfield) {
   for (int v = 0; v < nfield; v++) f[v * a.fs + o + cg] = f[v * a.fs + o + ce];
 }
 
-// 'cont' ghost cells of a 2-D patch (blockIdx.y = entry of the active list).  The reference
-// fills the x regions over the whole grid along y, then the y regions over
+// 'cont' ghost cells of a 2-D patch (blockIdx.y = entry of the active
+// list), for the initial fill.  The reference fills the x regions over the whole grid along y, then the y regions over
 // the whole grid along x (src/vacgrid.t:195-512), so every ghost cell ends
 // up with the mesh cell at its clamped indices.  The ghost frame of a
 // G0 x G1 grid: rows 0, 1, G1-2, G1-1 in full (4*G0 cells), then columns
