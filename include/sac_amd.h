@@ -254,7 +254,9 @@ int sac_batch_set_shape(sac_batch_handle h, int index);
  * y in cells (tubes: tbx = threads*ck, tby = 1; patches: ck = 1). */
 int sac_batch_shape(sac_batch_handle h, int *ck, int *tbx, int *tby);
 /* Advance every tube to its end, replaying a graph of steps_per_graph steps
- * at most max_steps steps in total; returns the steps issued.  Once at most
+ * at most max_steps steps in total; returns the steps issued.
+ * steps_per_graph < 0 launches the kernels of each step directly instead
+ * (same results; for tools that trace kernel launches).  Once at most
  * half of the tubes the tiles cover still run, the tiles and the graph are
  * rebuilt over the running ones (the results do not change). */
 int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,

@@ -152,3 +152,26 @@ class Ensemble:
         b = ctypes.c_longlong()
         _check(_lib().sac_batch_memory(self._h, ctypes.byref(b)))
         return b.value
+
+
+def _main():
+    """Run the seeded tube and patch ensembles (default tiles) for a few
+    steps with direct kernel launches (steps_per_graph = -1), e.g. under a
+    kernel tracer."""
+    import argparse
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--tubes", type=int, default=4096)
+    ap.add_argument("--patches", type=int, default=1024)
+    args = ap.parse_args()
+    for name, probs in (("tubes", riemann_ensemble(count=args.tubes)),
+                        ("patches", patch_ensemble(count=args.patches))):
+        e = Ensemble(probs, courantpar=0.3, tmax=0.1)
+        n = e.run(steps_per_graph=-1, max_steps=args.steps)
+        ck, tx, ty = e.shape()
+        e.close()
+        print(f"{name}: {len(probs)} problems, {n} steps, tile {tx}x{ty}", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -994,6 +994,26 @@ int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,
                   long long* steps_done) {
   if (!h) return bfail(SAC_E_ARG, "null handle");
   Batch& b = *(Batch*)h;
+  if (steps_per_graph < 0) {
+    // direct launches, one step at a time (for tools that trace launches)
+    long long done = 0;
+    while (done < max_steps) {
+      hipLaunchKernelGGL(kb_zero_u32, dim3(1), dim3(1), 0, b.stream, b.a.running);
+      enqueue_step(b);
+      unsigned int running = 0;
+      BCHK(hipMemcpyAsync(&running, b.d_running, sizeof(running), hipMemcpyDeviceToHost,
+                          b.stream));
+      BCHK(hipStreamSynchronize(b.stream));
+      done += 1;
+      if (running == 0) break;
+      if (2 * (size_t)running <= b.active.size()) {
+        if (int rc = compact(b)) return rc;
+      }
+    }
+    b.steps_run += done;
+    if (steps_done) *steps_done = done;
+    return 0;
+  }
   if (steps_per_graph < 1) steps_per_graph = 32;
   if (!b.graph || b.graph_steps != steps_per_graph) {
     if (int rc = build_graph(b, steps_per_graph)) return rc;

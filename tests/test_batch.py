@@ -311,3 +311,24 @@ def test_patch_ensemble_bitwise_equals_single_patch_grid_runs(mods):
         w1, dts = device.run(prob, steps=10 ** 6, tmax=tmax)
         assert it[i] == len(dts), (i, it[i], len(dts))
         np.testing.assert_array_equal(ws[i][:8], w1[:8], err_msg=f"patch {i}")
+
+
+def test_direct_launches_give_the_graph_run(mods):
+    """steps_per_graph < 0 launches each step's kernels directly (for kernel
+    tracers); the run is the graph run bit for bit."""
+    batch, _ = mods
+    for probs in (_small_ensemble(batch, count=5, seed=37), _small_patches(batch, count=3, seed=41)):
+        a = batch.Ensemble(probs, tmax=0.005)
+        a.run(steps_per_graph=8)
+        wa = a.download()
+        ta, ia, _ = a.status()
+        a.close()
+        b = batch.Ensemble(probs, tmax=0.005)
+        b.run(steps_per_graph=-1)
+        wb = b.download()
+        tb_, ib, _ = b.status()
+        b.close()
+        np.testing.assert_array_equal(ia, ib)
+        np.testing.assert_array_equal(ta, tb_)
+        for x, y in zip(wa, wb):
+            np.testing.assert_array_equal(x, y)
