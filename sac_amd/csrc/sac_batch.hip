@@ -715,7 +715,7 @@ int compact(Batch& b) {
   b.active.swap(act);
   const int steps = b.graph_steps;
   if (int rc = set_shape(b, b.shape)) return rc;
-  return build_graph(b, steps);
+  return steps > 0 ? build_graph(b, steps) : 0;  // no graph in direct-launch runs
 }
 
 }  // namespace
