@@ -33,10 +33,17 @@
  *                     oracle's own right-hand side.
  *   whole step        hydrostatic zero-perturbation stays exactly zero,
  *                     uniform state stays uniform, axis permutation.
- * The remaining composition (order of getboundary calls within a step, the
- * lagged maxviscoef in the dt) is restated from the source text and is
- * pinned only through those invariants: "parity partial" for the dt
- * sequence.
+ *   getdt_courant     courantpar / max(cmax_d/dx_d) with a textbook fast
+ *                     speed, on random and uniform states.
+ *   getdt_visc        0.25 min(dx)^2/maxviscoef for set values, the reset to
+ *                     0, no limit when nothing was collected; maxviscoef is
+ *                     the largest setnu coefficient (known-answer step); the
+ *                     first step of a run has only the courant limit and the
+ *                     second uses the value the first collected.
+ *   getdt_grav        1/sqrt(|g|/min(dx)) at it == itmin, kept afterwards,
+ *                     absent when the first call comes after itmin.
+ * The order of getboundary calls within a step is restated from the source
+ * text and pinned through the invariants and the integrator checks above.
  */
 #include "sac_oracle.h"
 
