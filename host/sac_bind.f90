@@ -44,6 +44,23 @@ module sac_bind
       real(c_double), intent(out) :: w(*)
     end function sac_dev_download
 
+    integer(c_int) function sac_dev_save_begin(handle) bind(C, name='sac_dev_save_begin')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+    end function sac_dev_save_begin
+
+    integer(c_int) function sac_dev_save_ready(handle, ready) bind(C, name='sac_dev_save_ready')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+      integer(c_int), intent(out) :: ready
+    end function sac_dev_save_ready
+
+    integer(c_int) function sac_dev_save_end(handle, w) bind(C, name='sac_dev_save_end')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: handle
+      real(c_double), intent(out) :: w(*)
+    end function sac_dev_save_end
+
     integer(c_int) function sac_dev_getboundary(handle) bind(C, name='sac_dev_getboundary')
       import :: c_int, c_ptr
       type(c_ptr), value :: handle

@@ -369,8 +369,16 @@ program sac_vac
   real(dp) :: time0, time1, dtold, dtc, cmax
   real(dp), allocatable :: qx(:), dxaxis(:)
   integer(c_int), allocatable :: iws(:), bcodes(:)
-  integer(c_int) :: nxc(3)
+  integer(c_int) :: nxc(3), ready
   type(c_ptr) :: h
+  ! asynchronous saves: the snapshot of the files due at one iteration is
+  ! copied off the device while the next steps run and written once it has
+  ! arrived, with the it, t and dt of the iteration that asked for it
+  ! (SAC_SYNC_SAVES=1 in the environment downloads and writes at once)
+  logical :: asyncsave, savepending, savedue(nfile)
+  integer :: itsnap
+  real(dp) :: tsnap, dtsnap
+  character(len=8) :: envval
 
   time0 = cputime()
 
@@ -484,14 +492,29 @@ program sac_vac
   call check(sac_dev_getboundary(h))
   call check(sac_dev_download(h, w))
   dtold = 0.0_dp
+  call get_environment_variable('SAC_SYNC_SAVES', envval)
+  asyncsave = trim(envval) /= '1'
+  savepending = .false.
   ! main loop (src/vac.t:55-106)
   do
     do ifile = 1, nfile
-      if (timetosave(ifile)) then
-        call check(sac_dev_download(h, w))
-        call savefile(ifile)
-      end if
+      savedue(ifile) = timetosave(ifile)
     end do
+    if (any(savedue)) then
+      if (asyncsave) then
+        call finish_save()
+        call check(sac_dev_save_begin(h))
+        savepending = .true.
+        itsnap = it
+        tsnap = t
+        dtsnap = dt
+      else
+        call check(sac_dev_download(h, w))
+        do ifile = 1, nfile
+          if (savedue(ifile)) call savefile(ifile)
+        end do
+      end if
+    end if
     if (dtpar > 0.0_dp) then
       dt = dtpar
     else
@@ -513,7 +536,12 @@ program sac_vac
     call check(sac_dev_advance(h, dt))
     it = it + 1
     t = t + dt
+    if (savepending) then
+      call check(sac_dev_save_ready(h, ready))
+      if (ready == 1) call finish_save()
+    end if
   end do
+  call finish_save()
   time1 = cputime() - time0
   call check(sac_dev_download(h, w))
   do ifile = 1, nfile
@@ -532,6 +560,27 @@ contains
     integer(c_int), intent(in) :: code
     if (code /= 0) call die('libsac_amd: ' // trim(sac_error_message()))
   end subroutine check
+
+  ! write the files of the pending asynchronous save (waits for its copy)
+  subroutine finish_save()
+    integer :: itnow
+    real(dp) :: tnow, dtnow
+    if (.not. savepending) return
+    savepending = .false.
+    call check(sac_dev_save_end(h, w))
+    itnow = it
+    tnow = t
+    dtnow = dt
+    it = itsnap
+    t = tsnap
+    dt = dtsnap
+    do ifile = 1, nfile
+      if (savedue(ifile)) call savefile(ifile)
+    end do
+    it = itnow
+    t = tnow
+    dt = dtnow
+  end subroutine finish_save
 
   integer(c_int) function bc_code(s)
     character(len=*), intent(in) :: s

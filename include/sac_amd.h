@@ -111,6 +111,19 @@ int sac_dev_free(sac_handle h);
 int sac_dev_upload(sac_handle h, const double *w);
 int sac_dev_download(sac_handle h, double *w);
 
+/* Asynchronous saves (the D2H of the state at save times, src/vac.t:56-58 ->
+ * savefile, src/vacio.t:901-958).  sac_dev_save_begin takes a device copy of
+ * the state on the handle's stream and starts its copy to pinned host memory
+ * on a second stream; later calls (sac_dev_advance ...) may run at once and
+ * overlap that copy.  sac_dev_save_ready reports (without waiting) whether
+ * the copy has finished; sac_dev_save_end waits for it and writes w exactly
+ * as sac_dev_download would have at the time of sac_dev_save_begin.  One
+ * save at a time per handle.  The device copy and the pinned buffer (one
+ * state each) are allocated at the first save and kept until sac_dev_free. */
+int sac_dev_save_begin(sac_handle h);
+int sac_dev_save_ready(sac_handle h, int *ready);
+int sac_dev_save_end(sac_handle h, double *w);
+
 /* Slot-level copies of the evolving variables (1..2+2*ndir) for stage-level
  * parity checks; the background part of w is ignored on upload and filled
  * from the device background on download. */
@@ -266,6 +279,11 @@ int sac_batch_status(sac_batch_handle h, double *t, int *it, int *state);
 /* The ensemble state in the layout of sac_batch_init(_nd)'s w. */
 int sac_batch_download(sac_batch_handle h, double *w);
 int sac_batch_memory(sac_batch_handle h, long long *bytes);
+/* Tile rebuilds so far (a run rebuilds the tiles over the running tubes
+ * once at most half of the covered ones still run) and the number of tubes
+ * the tiles cover now.  No counterpart in the reference (its runs are one
+ * tube per process); for callers that check the compaction schedule. */
+int sac_batch_compactions(sac_batch_handle h, int *compactions, int *covered);
 const char *sac_batch_last_error(void);
 
 

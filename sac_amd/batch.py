@@ -16,7 +16,7 @@ EXPORTS = [
     "sac_batch_init", "sac_batch_init_nd", "sac_batch_free", "sac_batch_tune", "sac_batch_shape",
     "sac_batch_set_shape",
     "sac_batch_run", "sac_batch_status", "sac_batch_download", "sac_batch_memory",
-    "sac_batch_last_error",
+    "sac_batch_compactions", "sac_batch_last_error",
 ]
 
 STATE = {0: "running", 1: "finished", 2: "failed", 3: "dt below dtmin"}
@@ -43,6 +43,7 @@ def _lib():
             "sac_batch_status": (I, [V, dp, ip, ip]),
             "sac_batch_download": (I, [V, dp]),
             "sac_batch_memory": (I, [V, llp]),
+            "sac_batch_compactions": (I, [V, ip, ip]),
             "sac_batch_last_error": (ctypes.c_char_p, []),
         }
         for name, (res, args) in sig.items():
@@ -152,6 +153,12 @@ class Ensemble:
         b = ctypes.c_longlong()
         _check(_lib().sac_batch_memory(self._h, ctypes.byref(b)))
         return b.value
+
+    def compactions(self):
+        """(tile rebuilds so far, tubes the tiles cover now)"""
+        n, c = ctypes.c_int(), ctypes.c_int()
+        _check(_lib().sac_batch_compactions(self._h, ctypes.byref(n), ctypes.byref(c)))
+        return n.value, c.value
 
 
 def _main():

@@ -113,6 +113,15 @@ struct Dev {
   long long t_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int xcd = 8;  // XCDs of the device (block renumbering of the line kernels)
   Closure* cl = nullptr;
+  // asynchronous saves (sac_dev_save_*): a device copy of W taken on the
+  // handle's stream, copied to pinned host memory on a second stream (both
+  // at the default priority) while later steps run
+  void* save_dev = nullptr;      // ne fields, storage precision
+  void* save_host = nullptr;     // pinned, ne * ncell_grid elements (host layout)
+  hipStream_t save_stream = nullptr;
+  hipEvent_t save_taken = nullptr, save_done = nullptr;
+  bool save_pending = false;
+  std::vector<double> bg_host;   // background of the last upload (static)
 };
 
 long long fieldsz(const Dev& d) { return d.g.fs; }
@@ -566,6 +575,12 @@ int sac_dev_free(sac_handle h) {
     hipFree(c->targ);
     delete c;
   }
+  if (d->save_stream) hipStreamSynchronize(d->save_stream);
+  hipFree(d->save_dev);
+  if (d->save_host) hipHostFree(d->save_host);
+  if (d->save_taken) hipEventDestroy(d->save_taken);
+  if (d->save_done) hipEventDestroy(d->save_done);
+  if (d->save_stream) hipStreamDestroy(d->save_stream);
   if (d->stream) hipStreamDestroy(d->stream);
   delete d;
   return 0;
@@ -578,6 +593,7 @@ int sac_dev_upload(sac_handle h, const double* w) {
   if (int rc = copy_in(d, d.reg[SAC_SLOT_W], w, d.ne, d.esz)) return rc;
   if (int rc = copy_in(d, d.bg, w + (size_t)d.ne * d.ncell_grid, d.nbg, sizeof(double))) return rc;
   d.red_valid[SAC_SLOT_W] = false;
+  d.bg_host.clear();  // the ghosts of the background change at the next getboundary
   return 0;
 }
 
@@ -603,6 +619,77 @@ int sac_dev_download_slot(sac_handle h, int slot, double* w) {
   Dev& d = *(Dev*)h;
   if (int rc = copy_out(d, w, d.reg[slot], d.ne, d.esz)) return rc;
   return copy_out(d, w + (size_t)d.ne * d.ncell_grid, d.bg, d.nbg, sizeof(double));
+}
+
+// ---------------------------------------------------------------------
+// asynchronous saves: the snapshot is a device-to-device copy of W on the
+// handle's stream (so later steps may overwrite W at once), copied to pinned
+// host memory on the save stream after an event; sac_dev_save_end waits for
+// that copy and writes the caller's array.
+
+int sac_dev_save_begin(sac_handle h) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  if (d.save_pending) return fail(SAC_E_ARG, "a save is already in progress");
+  const size_t fbytes = d.esz * (size_t)d.g.fs;
+  if (!d.save_stream) {
+    HIPCHK(hipStreamCreateWithFlags(&d.save_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&d.save_taken, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&d.save_done, hipEventDisableTiming));
+    if (int rc = balloc(d, &d.save_dev, fbytes * (size_t)d.ne)) return rc;
+    HIPCHK(hipHostMalloc(&d.save_host, d.esz * (size_t)d.ncell_grid * (size_t)d.ne,
+                         hipHostMallocDefault));
+  }
+  if (d.bg_host.empty()) {  // the background is static: one synchronous copy
+    d.bg_host.resize((size_t)d.nbg * (size_t)d.ncell_grid);
+    if (int rc = copy_out(d, d.bg_host.data(), d.bg, d.nbg, sizeof(double))) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(d.save_dev, d.reg[SAC_SLOT_W], fbytes * (size_t)d.ne,
+                        hipMemcpyDeviceToDevice, d.stream));
+  HIPCHK(hipEventRecord(d.save_taken, d.stream));
+  HIPCHK(hipStreamWaitEvent(d.save_stream, d.save_taken, 0));
+  const size_t w = d.esz * (size_t)d.g.G[0];
+  const size_t spitch = d.esz * (size_t)d.g.st[1];
+  const size_t rows = (size_t)d.g.G[1] * (size_t)d.g.G[2];
+  for (int f = 0; f < d.ne; f++) {
+    char* dst = (char*)d.save_host + d.esz * (size_t)f * (size_t)d.ncell_grid;
+    const char* src = (const char*)d.save_dev + fbytes * (size_t)f;
+    HIPCHK(hipMemcpy2DAsync(dst, w, src, spitch, w, rows, hipMemcpyDeviceToHost,
+                            d.save_stream));
+  }
+  HIPCHK(hipEventRecord(d.save_done, d.save_stream));
+  d.save_pending = true;
+  return 0;
+}
+
+int sac_dev_save_ready(sac_handle h, int* ready) {
+  if (int rc = check_handle(h)) return rc;
+  if (!ready) return fail(SAC_E_ARG, "ready is null");
+  Dev& d = *(Dev*)h;
+  if (!d.save_pending) return fail(SAC_E_ARG, "no save in progress");
+  const hipError_t e = hipEventQuery(d.save_done);
+  if (e != hipSuccess && e != hipErrorNotReady)
+    return fail(SAC_E_HIP, "hipEventQuery: %s", hipGetErrorString(e));
+  *ready = (e == hipSuccess) ? 1 : 0;
+  return 0;
+}
+
+int sac_dev_save_end(sac_handle h, double* w) {
+  if (int rc = check_handle(h)) return rc;
+  if (!w) return fail(SAC_E_ARG, "w is null");
+  Dev& d = *(Dev*)h;
+  if (!d.save_pending) return fail(SAC_E_ARG, "no save in progress");
+  d.save_pending = false;
+  HIPCHK(hipEventSynchronize(d.save_done));
+  const size_t n = (size_t)d.ne * (size_t)d.ncell_grid;
+  if (d.esz == sizeof(double)) {
+    memcpy(w, d.save_host, sizeof(double) * n);
+  } else {
+    const float* s = (const float*)d.save_host;
+    for (size_t i = 0; i < n; i++) w[i] = (double)s[i];
+  }
+  memcpy(w + n, d.bg_host.data(), sizeof(double) * d.bg_host.size());
+  return 0;
 }
 
 int sac_dev_getboundary(sac_handle h) {
@@ -633,6 +720,7 @@ int sac_dev_getboundary(sac_handle h) {
       sac::launch_ghost_store(st, d.snap, d.stream);
     }
     d.initialized = true;
+    d.bg_host.clear();  // background ghosts filled: the cached copy is stale
   } else {
     ghost_fill_evolving(d, SAC_SLOT_W);
   }

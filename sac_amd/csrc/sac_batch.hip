@@ -493,6 +493,7 @@ struct Batch {
   int ntiles = 0;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
+  int compactions = 0;  // tile rebuilds over the running tubes so far
   long long bytes = 0;
   long long steps_run = 0;
 
@@ -514,6 +515,14 @@ int balloc(Batch& b, void** p, size_t bytes) {
 constexpr size_t kFewPatches = 64;
 
 int set_shape(Batch& b, Shape s) {
+  // the graph's kernel nodes hold the tile tables freed below: destroy the
+  // executable graph first (the stream is idle here), so no graph ever
+  // refers to freed memory
+  if (b.graph) {
+    hipGraphExecDestroy(b.graph);
+    b.graph = nullptr;
+  }
+  b.graph_steps = 0;
   std::vector<Tile> tiles;
   if (b.ndim == 1) {
     const int span = s.ck * s.tb;
@@ -572,10 +581,6 @@ int set_shape(Batch& b, Shape s) {
                  hipMemcpyHostToDevice));
   b.a.active = b.d_active;
   b.a.nactive = (int)b.active.size();
-  if (b.graph) {
-    hipGraphExecDestroy(b.graph);
-    b.graph = nullptr;
-  }
   return 0;
 }
 
@@ -653,7 +658,7 @@ void enqueue_cmax(Batch& b, int slot) {
     hipLaunchKernelGGL((kb_cmax<1, 1>), gt, bt, 0, b.stream, b.a, slot);
 }
 
-// capture `steps` time steps (plus the running-tube counter reset) in one graph
+// capture `steps` time steps (and the running-tube counter reset) in one graph
 int build_graph(Batch& b, int steps) {
   if (b.graph) {
     hipGraphExecDestroy(b.graph);
@@ -661,8 +666,12 @@ int build_graph(Batch& b, int steps) {
   }
   hipGraph_t g = nullptr;
   BCHK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(kb_zero_u32, dim3(1), dim3(1), 0, b.stream, b.a.running);
-  for (int s = 0; s < steps; s++) enqueue_step(b);
+  // the running-tube counter is cleared before the last step only, so after
+  // a replay it holds the number of tubes that took that step (kb_tick)
+  for (int s = 0; s < steps; s++) {
+    if (s == steps - 1) hipLaunchKernelGGL(kb_zero_u32, dim3(1), dim3(1), 0, b.stream, b.a.running);
+    enqueue_step(b);
+  }
   BCHK(hipStreamEndCapture(b.stream, &g));
   hipError_t e = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
@@ -705,7 +714,7 @@ int restore_state(Batch& b, const RunState& r) {
 
 // drop finished tubes from the tiles (their blocks would only exit), then
 // recapture the graph; the arithmetic of the running tubes is unchanged
-int compact(Batch& b) {
+int compact(Batch& b, int steps_per_graph) {
   std::vector<int> st(b.ntubes);
   BCHK(hipMemcpy(st.data(), b.d_state, sizeof(int) * b.ntubes, hipMemcpyDeviceToHost));
   std::vector<int> act;
@@ -713,9 +722,10 @@ int compact(Batch& b) {
     if (st[i] == RUNNING) act.push_back(i);
   if (act.empty() || act.size() == b.active.size()) return 0;
   b.active.swap(act);
-  const int steps = b.graph_steps;
   if (int rc = set_shape(b, b.shape)) return rc;
-  return steps > 0 ? build_graph(b, steps) : 0;  // no graph in direct-launch runs
+  b.compactions += 1;
+  // graph runs recapture; direct-launch runs (steps_per_graph <= 0) never capture
+  return steps_per_graph > 0 ? build_graph(b, steps_per_graph) : 0;
 }
 
 }  // namespace
@@ -1007,7 +1017,7 @@ int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,
       done += 1;
       if (running == 0) break;
       if (2 * (size_t)running <= b.active.size()) {
-        if (int rc = compact(b)) return rc;
+        if (int rc = compact(b, -1)) return rc;
       }
     }
     b.steps_run += done;
@@ -1028,7 +1038,7 @@ int sac_batch_run(sac_batch_handle h, int steps_per_graph, long long max_steps,
     if (running == 0) break;
     // once at most half of the covered tubes run, rebuild the tiles
     if (2 * (size_t)running <= b.active.size()) {
-      if (int rc = compact(b)) return rc;
+      if (int rc = compact(b, steps_per_graph)) return rc;
     }
   }
   b.steps_run += done;
@@ -1068,6 +1078,14 @@ int sac_batch_download(sac_batch_handle h, double* w) {
 int sac_batch_memory(sac_batch_handle h, long long* bytes) {
   if (!h) return bfail(SAC_E_ARG, "null handle");
   if (bytes) *bytes = ((Batch*)h)->bytes;
+  return 0;
+}
+
+int sac_batch_compactions(sac_batch_handle h, int* compactions, int* covered) {
+  if (!h) return bfail(SAC_E_ARG, "null handle");
+  const Batch& b = *(const Batch*)h;
+  if (compactions) *compactions = b.compactions;
+  if (covered) *covered = (int)b.active.size();
   return 0;
 }
 

@@ -28,7 +28,8 @@ EXPORTS = [
     "sac_dev_getdt_courant", "sac_dev_getdt_special", "sac_dev_advance",
     "sac_dev_advect1", "sac_dev_set_divbfix", "sac_dev_set_resistive", "sac_dev_copy_slot", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
     "sac_dev_timing", "sac_dev_timing_get", "sac_dev_synchronize",
-    "sac_dev_memory", "sac_dev_last_error",
+    "sac_dev_memory", "sac_dev_save_begin", "sac_dev_save_ready", "sac_dev_save_end",
+    "sac_dev_last_error",
 ]
 
 
@@ -72,6 +73,9 @@ def load_library(path: str = LIB_PATH):
         "sac_dev_timing_get": (I, [V, I, dp, ctypes.POINTER(ctypes.c_longlong)]),
         "sac_dev_synchronize": (I, [V]),
         "sac_dev_memory": (I, [V, ctypes.POINTER(ctypes.c_longlong)]),
+        "sac_dev_save_begin": (I, [V]),
+        "sac_dev_save_ready": (I, [V, ip]),
+        "sac_dev_save_end": (I, [V, dp]),
         "sac_dev_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -168,6 +172,21 @@ class Device:
     def download(self):
         out = np.zeros(self.shape)
         _check(load_library().sac_dev_download(self._h, _dp(out)))
+        return out
+
+    def save_begin(self):
+        """Start an asynchronous copy of the state to host memory."""
+        _check(load_library().sac_dev_save_begin(self._h))
+
+    def save_ready(self):
+        r = ctypes.c_int()
+        _check(load_library().sac_dev_save_ready(self._h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def save_end(self):
+        """Wait for the copy started by save_begin; the state at that time."""
+        out = np.zeros(self.shape)
+        _check(load_library().sac_dev_save_end(self._h, _dp(out)))
         return out
 
     def upload_slot(self, slot, w):

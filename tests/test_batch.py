@@ -332,3 +332,31 @@ def test_direct_launches_give_the_graph_run(mods):
         np.testing.assert_array_equal(ta, tb_)
         for x, y in zip(wa, wb):
             np.testing.assert_array_equal(x, y)
+
+
+def test_graph_runs_compact_the_tiles(mods):
+    """Tubes of 30-160 cells take very different numbers of steps to the same
+    tmax, so a graph run rebuilds its tiles over the running tubes once at
+    most half of the covered ones still run (the counter read after a replay
+    holds the tubes of its last step); the run equals the direct-launch run,
+    which compacts on the same rule step by step."""
+    batch, _ = mods
+    tubes = _small_ensemble(batch, count=10, seed=7)
+    a = batch.Ensemble(tubes, tmax=0.02)
+    a.run(steps_per_graph=4)
+    n, covered = a.compactions()
+    wa = a.download()
+    ia = a.status()[1]
+    a.close()
+    assert n >= 1 and covered < len(tubes), (n, covered)
+    b = batch.Ensemble(tubes, tmax=0.02)
+    b.tune(trial_steps=2)  # leaves no graph behind for the direct-launch run
+    b.run(steps_per_graph=-1)
+    nb, _ = b.compactions()
+    wb = b.download()
+    ib = b.status()[1]
+    b.close()
+    assert nb >= 1
+    np.testing.assert_array_equal(ia, ib)
+    for x, y in zip(wa, wb):
+        np.testing.assert_array_equal(x[:8], y[:8])

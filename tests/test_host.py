@@ -175,3 +175,28 @@ def test_host_threestep_divbfix_extrapolating_boundaries(exe, tmp_path, oracle_m
     got[(Ellipsis,) + prob.mesh_slices()] = snaps[-1].w
     errs = per_var_errors(prob, got, ref, 8)
     assert max(errs) <= 1e-12, errs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("out_fmt", ["binary", "ascii"])
+def test_host_async_saves_equal_sync_saves(exe, tmp_path, out_fmt):
+    """Saves copy the state off the device while the next steps run
+    (sac_dev_save_begin/ready/end) and write it later with the it, t and dt
+    of the iteration that asked for it; the snapshot and log files are those
+    of immediate downloads (SAC_SYNC_SAVES=1) byte for byte, for saves every
+    step (each one still in flight when the next is due) and every 3 steps."""
+    prob = problems.stratified(n=16, nxyz=(16, 12, 10))
+    for dits in ((1, 1), (3, 2)):
+        outs = []
+        for env in ({}, {"SAC_SYNC_SAVES": "1"}):
+            d = tmp_path / f"{dits[0]}_{len(env)}"
+            d.mkdir()
+            par = _setup(d, prob, fmt="binary", out_fmt=out_fmt, itmax=7, ditsave=dits)
+            r = subprocess.run([exe], input=par, capture_output=True, text=True, cwd=d,
+                               timeout=300, env=dict(os.environ, **env))
+            assert r.returncode == 0, r.stdout + r.stderr
+            outs.append(((d / f"out.{out_fmt}").read_bytes(), (d / "run.log").read_bytes()))
+        assert outs[0] == outs[1], dits
+    snaps = (vacio.read_binary if out_fmt == "binary" else vacio.read_ascii)(
+        tmp_path / f"3_0/out.{out_fmt}")
+    assert [s.it for s in snaps] == [0, 2, 4, 6, 7]
