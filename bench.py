@@ -64,11 +64,10 @@ def cpu_baseline(budget_s: float):
     """Time the C oracle on a bounded sample of the same workload."""
     from oracle import oracle as om
     from sac_amd import problems
-    from tests.helpers import make_oracle
     om.build()
     n = 32
     prob = problems.stratified(n=n)
-    o = make_oracle(prob, om)
+    o = om.from_problem(prob)
     o.step(prob.courantpar)  # first step (no dt_visc history) untimed
     t0 = time.perf_counter()
     steps = 0
@@ -130,7 +129,7 @@ def main():
               "supported (multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         return 2
     from sac_amd import device, problems
-    from tests.helpers import per_var_errors
+    from sac_amd.errors import per_var_errors
 
     n = args.n
     el, tim, w64, calib, finite = timed_run(device, problems, n, args.steps, args.warmup,
@@ -304,7 +303,7 @@ def learned_leg(n, steps):
     the speed ratio of the two runs."""
     import numpy as np
     from sac_amd import closure, device, problems
-    from tests.helpers import per_var_errors
+    from sac_amd.errors import per_var_errors
     prob = problems.stratified(n=n, amp=0.1)
     ncell = n ** 3
 

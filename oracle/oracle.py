@@ -278,3 +278,16 @@ class Oracle:
             ax = 2 - d  # reference axis index 0..2
             sl.append(slice(2, -2) if ax < self.ndim else slice(None))
         return a[(Ellipsis,) + tuple(sl)]
+
+
+def from_problem(prob):
+    """An oracle set up with a sac_amd Problem (grid, eqpar, boundaries,
+    state), with getboundary applied and the clock at it=itmin=0, t=0."""
+    from sac_amd import problems
+    o = Oracle(prob.ndim, prob.ndir, prob.nx, prob.eqpar, prob.typeB,
+               grav0_=problems.GRAV0, nu_=problems.NU, iws=prob.iws())
+    o.set_x(prob.x.reshape(-1))
+    o.set_w(prob.w)
+    o.getboundary()
+    o.set_time(0, 0, 0.0)
+    return o

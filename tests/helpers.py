@@ -5,39 +5,17 @@ from __future__ import annotations
 import numpy as np
 
 from sac_amd import problems
+from sac_amd.errors import mesh, per_var_errors, per_var_maxabs, rel_l2  # noqa: F401
 
 
 def make_oracle(prob, oracle_mod):
-    o = oracle_mod.Oracle(prob.ndim, prob.ndir, prob.nx, prob.eqpar, prob.typeB,
-                          grav0_=problems.GRAV0, nu_=problems.NU, iws=prob.iws())
-    o.set_x(prob.x.reshape(-1))
-    o.set_w(prob.w)
-    o.getboundary()
-    o.set_time(0, 0, 0.0)
-    return o
+    return oracle_mod.from_problem(prob)
 
 
 def oracle_run(prob, oracle_mod, steps, typeadvance="fourstep"):
     o = make_oracle(prob, oracle_mod)
     dts = [o.step(prob.courantpar, typeadvance) for _ in range(steps)]
     return o.get_w(), dts, o
-
-
-def mesh(prob, a):
-    return a[(Ellipsis,) + prob.mesh_slices()]
-
-
-def rel_l2(a, b):
-    """rel-L2 of a against b; absolute L2 when b is identically zero"""
-    nb = np.linalg.norm(b)
-    d = np.linalg.norm(a - b)
-    return d / nb if nb > 0 else d
-
-
-def per_var_errors(prob, a, b, nvar=None):
-    nvar = nvar or a.shape[0]
-    ma, mb = mesh(prob, a), mesh(prob, b)
-    return [rel_l2(ma[i], mb[i]) for i in range(nvar)]
 
 
 def assert_close_state(prob, got, ref, tol, nvar=None, what=""):

@@ -360,3 +360,78 @@ def test_graph_runs_compact_the_tiles(mods):
     np.testing.assert_array_equal(ia, ib)
     for x, y in zip(wa, wb):
         np.testing.assert_array_equal(x[:8], y[:8])
+
+
+def _fma_floor(prob, dts, w_ref):
+    """per-variable rel-L2 between the oracle and the same oracle built with
+    FMA contraction, both advanced with dts: the reproducibility floor of
+    the reference algorithm on this run"""
+    from tests.helpers import per_var_errors
+    from tests.test_dt_conditioning import _fma_build, _run_problem
+    wf = _run_problem(_fma_build(), prob, dts)
+    return per_var_errors(prob, wf, w_ref, 8)
+
+
+@pytest.mark.parametrize("which", ["tubes", "patches"])
+def test_members_to_tmax_match_oracle_under_their_dt(mods, oracle_mod, which):
+    """Whole runs to t=0.1 of the three smallest members of the seeded
+    4096-tube and 1024-patch ensembles against the oracle advanced with the
+    member's own dt sequence.  A member of an ensemble equals the same
+    member run alone through sac_dev_* bit for bit, dt sequence included
+    (tested above), so the single run supplies the sequence.  Bar: 1e-12 per
+    variable, or 4x the oracle's own FMA/no-FMA spread under the same dt
+    where that spread is larger (it grows with the number of steps across
+    shocks)."""
+    batch, device = mods
+    from tests.helpers import make_oracle, per_var_errors
+    members = batch.riemann_ensemble(count=4096) if which == "tubes" else \
+        batch.patch_ensemble(count=1024)
+    sizes = np.array([int(np.prod(p.nx)) for p in members])
+    for i in np.argsort(sizes)[:3]:
+        prob = members[i]
+        w1, dts = device.run(prob, tmax=0.1)
+        o = make_oracle(prob, oracle_mod)
+        for dt in dts:
+            o.advance(dt)
+        ref = o.get_w()
+        errs = per_var_errors(prob, w1, ref, 8)
+        floor = _fma_floor(prob, dts, ref)
+        bars = [max(1e-12, 4.0 * f) for f in floor]
+        print(f"{which} {i} ({prob.nx}, {len(dts)} steps): rel-L2 " +
+              " ".join(f"{e:.1e}" for e in errs) + " | floor " +
+              " ".join(f"{f:.1e}" for f in floor))
+        assert all(e <= b for e, b in zip(errs, bars)), (i, errs, bars)
+
+
+@pytest.mark.parametrize("which", ["tubes", "patches"])
+def test_ensemble_endings_match_oracle(mods, which):
+    """The seeded 4096-tube (1024-patch) ensemble run to t=0.1 on the GPU
+    against the oracle's run of every member on its own
+    (tests/golden/ensemble_{which}_oracle.npz, make_ensemble_oracle.py):
+    the members that stop at dt < dtmin (they blow up: maxviscoef diverges)
+    are the same on both sides, and so is the number of steps every member
+    takes, up to the rounding-level dt differences of the viscous limit in
+    flat regions (a few steps in a thousand)."""
+    import os
+    batch, _ = mods
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        f"ensemble_{which}_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} has not been generated")
+    ref = np.load(path)
+    members = batch.riemann_ensemble(count=4096) if which == "tubes" else \
+        batch.patch_ensemble(count=1024)
+    ens = batch.Ensemble(members, courantpar=0.3, tmax=0.1)
+    ens.run(steps_per_graph=32)
+    _, it, st = ens.status()
+    ens.close()
+    stopped = set(np.nonzero(st == 3)[0].tolist())
+    stopped_ref = set(np.nonzero(ref["state"] == 3)[0].tolist())
+    print(f"{which}: GPU stopped {sorted(stopped)}; oracle stopped {sorted(stopped_ref)}")
+    ok = (st == 1) & (ref["state"] == 1)
+    rel = np.abs(it[ok] - ref["it"][ok]) / ref["it"][ok]
+    print(f"{which}: steps of finished members: max rel. difference {rel.max():.2e}, "
+          f"{int((it[ok] != ref['it'][ok]).sum())} of {int(ok.sum())} differ")
+    assert stopped == stopped_ref
+    assert (st != 2).all()
+    assert rel.max() <= 0.01
