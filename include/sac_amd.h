@@ -139,6 +139,23 @@ int sac_dev_getboundary(sac_handle h);
  * into the last stage that produced the state. */
 int sac_dev_getdt_courant(sac_handle h, double courantpar, double *dt, double *courantmax);
 
+/* sac_dev_getdt_courant without a handle and without any device allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid it).  The caller passes device pointers in the reference layout
+ * (unpadded, index 1 fastest, two ghost layers per active axis):
+ *   w    the 2+2*ndir evolving fields (double, or float for SAC_PREC_MIXED)
+ *   bg   the 2+ndir background fields eb, rhob, bg1.. (double)
+ *   dx   the cell widths dx_1[ixG1], dx_2[ixG2], dx_3[ixG3] (double)
+ * and the stream to use (a hipStream_t, or null for the null stream).  The
+ * grid may hold at most 2^36 cells.  The reduction slots are a static device
+ * array of 6 uint64 per GPU, kept until the process ends: calls that
+ * overlap on one GPU race.  dt and courantmax are returned on the host, as
+ * by sac_dev_getdt_courant, and equal its values on the same state. */
+int sac_getdt_courant_noalloc(int ndim, int ndir, const int *nx, int precision, const void *w,
+                              const double *bg, const double *dx, double gamma,
+                              double courantpar, void *stream, double *dt, double *courantmax);
+
 /* getdt_special of the gravity+hyperdiffusion user module
  * (src/vacusr.f:1403-1421): dt = min(dt, getdt_visc, getdt_grav) with
  * getdt_visc (src/vacusr.viscosity.t:509-541) using the maxviscoef gathered

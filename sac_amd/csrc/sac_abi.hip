@@ -358,6 +358,19 @@ double red_host(Dev& d, int slot, int k) {
 
 }  // namespace
 
+namespace sac {
+// the message of sac_dev_last_error() for entry points in other files
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+}  // namespace sac
+
 extern "C" {
 
 const char* sac_dev_last_error(void) { return g_err.c_str(); }

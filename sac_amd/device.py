@@ -29,7 +29,7 @@ EXPORTS = [
     "sac_dev_advect1", "sac_dev_set_divbfix", "sac_dev_set_resistive", "sac_dev_copy_slot", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
     "sac_dev_timing", "sac_dev_timing_get", "sac_dev_synchronize",
     "sac_dev_memory", "sac_dev_save_begin", "sac_dev_save_ready", "sac_dev_save_end",
-    "sac_dev_last_error",
+    "sac_getdt_courant_noalloc", "sac_dev_last_error",
 ]
 
 
@@ -76,6 +76,7 @@ def load_library(path: str = LIB_PATH):
         "sac_dev_save_begin": (I, [V]),
         "sac_dev_save_ready": (I, [V, ip]),
         "sac_dev_save_end": (I, [V, dp]),
+        "sac_getdt_courant_noalloc": (I, [I, I, ip, I, V, V, V, D, D, V, dp, dp]),
         "sac_dev_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
