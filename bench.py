@@ -11,7 +11,8 @@ resident in HBM when the timed region starts.
 
 Metric: Mcell-updates/s per RK stage = N_mesh * steps * 4 / wall time.
 Roofline: the unit is one RK stage, which the library issues as a fixed
-kernel sequence (k_line for each axis, k_final, k_ghost); algorithmic bytes =
+kernel sequence (k_plane for the x and y work, k_line for z, k_final,
+k_ghost); algorithmic bytes =
 296 B per mesh cell per stage (4*n_evolve*8 + n_background*8 for 8 evolving
 and 5 background FP64 fields) x N_mesh, divided by the summed average
 durations of that sequence, measured with HIP events on the library's
@@ -115,7 +116,7 @@ def timed_run(device, problems, n, steps, warmup, precision, calib_copy=False):
     dev.synchronize()
     el = time.perf_counter() - t0
     tim = {k: dev.timing_get(getattr(device, k)) for k in
-           ("K_FINAL", "K_AXIS", "K_GHOST", "K_AXIS1", "K_AXIS2", "K_AXIS3")}
+           ("K_FINAL", "K_AXIS", "K_GHOST", "K_AXIS1", "K_AXIS2", "K_AXIS3", "K_PLANE")}
     dev.timing(False)
     w = dev.download()
     dev.close()
@@ -137,11 +138,12 @@ def main():
     f_ms, f_n = tim["K_FINAL"]
     a_ms, a_n = tim["K_AXIS"]
     g_ms, g_n = tim["K_GHOST"]
+    p_ms, p_n = tim["K_PLANE"]
     ncell = n ** 3
     stages = args.steps * 4
     value = ncell * stages / el / 1e6
-    # kernel time of one stage: 3 k_axis + 1 k_final + 1 k_ghost launches
-    per_stage_ms = (a_ms + f_ms + g_ms) / max(f_n, 1)
+    # kernel time of one stage: k_plane (x and y) + k_line (z) + k_final + k_ghost
+    per_stage_ms = (p_ms + a_ms + f_ms + g_ms) / max(f_n, 1)
     alg_bytes = BYTES_PER_CELL_STAGE["fp64"] * ncell
     achieved = alg_bytes / (per_stage_ms / 1e3) / 1e9
     stage_wall_s = el / stages
@@ -165,16 +167,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "RK stage = 3 x k_line<3,3,K> + k_final<3,3> + k_ghost",
+                     "kernel": "RK stage = k_plane (x and y) + k_line<3,3,2> (z) + k_final<3,3> "
+                               "+ k_ghost",
                      "avg_ms": round(per_stage_ms, 4),
                      "launches": f_n,
                      # the measured HBM traffic (rocprof, copy-calibrated) over the
                      # same time: how close the kernels run to the memory system
                      "traffic_rate_GBs": (round(traffic / 1e9 / (per_stage_ms / 1e3), 1)
                                           if traffic else None)},
-        "stage_breakdown_ms": {"k_axis": round(a_ms / max(a_n, 1), 4),
-                               "k_axis_per_axis": [round(tim[k][0] / max(tim[k][1], 1), 4)
-                                                   for k in ("K_AXIS1", "K_AXIS2", "K_AXIS3")],
+        "stage_breakdown_ms": {"k_plane": round(p_ms / max(p_n, 1), 4),
+                               "k_line_z": round(tim["K_AXIS3"][0] / max(tim["K_AXIS3"][1], 1), 4),
                                "k_final": round(f_ms / max(f_n, 1), 4),
                                "k_ghost": round(g_ms / max(g_n, 1), 4),
                                "stage_wall": round(stage_wall_s * 1e3, 4)},
@@ -190,7 +192,8 @@ def main():
         fm, fmn = timm["K_FINAL"]
         am, _ = timm["K_AXIS"]
         gm, _ = timm["K_GHOST"]
-        st_ms = (am + fm + gm) / max(fmn, 1)
+        pm, _ = timm["K_PLANE"]
+        st_ms = (pm + am + fm + gm) / max(fmn, 1)
         ach_m = BYTES_PER_CELL_STAGE["mixed"] * ncell / (st_ms / 1e3) / 1e9
         out["forms"] = {"mixed": {
             "value": round(ncell * stages / elm / 1e6, 3), "unit": "Mcell-updates/s",
@@ -207,6 +210,9 @@ def main():
         out.setdefault("forms", {})["batched_patches_2d"] = patches_leg(args.patches)
     if not args.no_learned:
         out.setdefault("forms", {})["learned_closure"] = learned_leg(n, args.learned_steps)
+    acc = accuracy_leg()
+    if acc is not None:
+        out["accuracy_vs_oracle"] = acc
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
@@ -294,33 +300,54 @@ def patches_leg(count):
 
 def learned_leg(n, steps):
     """The learned hyperdiffusion closure on the stratified setup at
-    amplitude 0.1: a plain FP64 run of `steps` RK4 steps writes (stencil,
-    exact ratio) samples of every face of two of its states (~1e8 at 256^3),
-    the network is trained on the GPU (4000 Adam steps of 16384 samples),
-    then the same `steps` are run with the closure on (shock-sensor threshold
-    0.1), with the plain run's dt sequence, and both are timed.  Reported:
-    per-variable rel-L2 of the learned run against the plain FP64 run and
-    the speed ratio of the two runs."""
+    amplitude 0.1 (n^3).  The network is trained on the GPU (4000 Adam steps
+    of 16384 samples) on ~1e8 (stencil, exact setnu ratio) pairs collected
+    from runs of the plain FP64 solver on the other configurations: the
+    Brio-Wu tube (800 cells, every 10th state of its run to t=0.1), the
+    stratified atmosphere at 128^3 (every 25th of 200 steps) and at 256^3
+    after 20 steps, both at amplitude 1e-3.  Then `steps` RK4 steps of the
+    amplitude-0.1 run are taken with the plain method and with the closure
+    on (shock-sensor threshold 0.1) under the plain run's dt sequence, both
+    timed.  Reported: per-variable rel-L2 of the learned run against the
+    plain FP64 run and the speed ratio of the two."""
     import numpy as np
     from sac_amd import closure, device, problems
     from sac_amd.errors import per_var_errors
     prob = problems.stratified(n=n, amp=0.1)
     ncell = n ** 3
 
-    def fresh():
-        d = device.Device(prob)
-        d.upload(prob.w)
+    def fresh(p):
+        d = device.Device(p)
+        d.upload(p.w)
         d.getboundary()
         return d
 
-    dev = fresh()
-    cl = closure.Closure(dev, capacity=2 * prob.ndim * ncell, max_blocks=256, seed=1)
-    dts = []
-    for i in range(steps):
-        dts.append(dev.step(0.3))
-        if i in (steps // 3, 2 * steps // 3):
-            cl.collect()
-    dev.synchronize()
+    train_dev = device.Device(prob)
+    cap = 800 * 200 + 8 * 3 * 128 ** 3 + 3 * 256 ** 3
+    cl = closure.Closure(train_dev, capacity=cap, max_blocks=256, seed=1)
+    sources = []
+    d1 = fresh(problems.brio_wu(n=800))
+    t, k = 0.0, 0
+    while t < 0.1:
+        t += d1.step(0.3)
+        k += 1
+        if k % 10 == 0:
+            cl.collect_from(d1)
+    d1.close()
+    sources.append(f"Brio-Wu 800 cells, {k // 10} states")
+    d2 = fresh(problems.stratified(n=128))
+    for k in range(1, 201):
+        d2.step(0.3)
+        if k % 25 == 0:
+            cl.collect_from(d2)
+    d2.close()
+    sources.append("stratified 128^3 amplitude 1e-3, 8 states over 200 steps")
+    d3 = fresh(problems.stratified(n=256))
+    for _ in range(20):
+        d3.step(0.3)
+    cl.collect_from(d3)
+    d3.close()
+    sources.append("stratified 256^3 amplitude 1e-3, the state after 20 steps")
     t0 = time.perf_counter()
     losses = [cl.train(1, lr=1e-3)]
     for k in range(8):
@@ -328,33 +355,39 @@ def learned_leg(n, steps):
     train_s = time.perf_counter() - t0
     samples = cl.n
     params = cl.params
-    dev.close()
+    train_dev.close()
 
-    def timed(learned):
-        d = fresh()
+    def timed(learned, dts=None):
+        d = fresh(prob)
         if learned:
             c = closure.Closure(d, capacity=0, max_blocks=8, seed=1)
             c.params = params
             c.enable(True, 0.1)
-        d.advance(dts[0])  # warm-up of the launch path (not timed)
+        d.advance(dts[0] if dts else 1e-6)  # warm-up of the launch path (not timed)
         d.close()
-        d = fresh()
+        d = fresh(prob)
         if learned:
             c = closure.Closure(d, capacity=0, max_blocks=8, seed=1)
             c.params = params
             c.enable(True, 0.1)
         d.synchronize()
+        out = []
         t0 = time.perf_counter()
-        for dt in dts:
-            d.advance(dt)
+        if dts is None:
+            for _ in range(steps):
+                out.append(d.step(0.3))
+        else:
+            for dt in dts:
+                d.advance(dt)
         d.synchronize()
         el = time.perf_counter() - t0
         w = d.download()
         d.close()
-        return el, w
+        return el, w, out
 
-    el_exact, w_exact = timed(False)
-    el_learned, w_learned = timed(True)
+    _, _, dts = timed(False)  # the plain run's dt sequence
+    el_exact, w_exact, _ = timed(False, dts)
+    el_learned, w_learned, _ = timed(True, dts)
     errs = per_var_errors(prob, w_learned, w_exact, 8)
     return {"value": round(ncell * steps * 4 / el_learned / 1e6, 3), "unit": "Mcell-updates/s",
             "workload": f"stratified {n}^3, perturbation amplitude 0.1 c_s, {steps} RK4 steps with "
@@ -362,9 +395,42 @@ def learned_leg(n, steps):
             "speedup_vs_fp64": round(el_exact / el_learned, 3),
             "rel_l2_vs_fp64_gpu": [float("%.3e" % e) for e in errs],
             "finite": bool(np.isfinite(w_learned).all()),
-            "training": {"samples": int(samples), "adam_steps": 4000, "batch": 16384,
-                         "seconds": round(train_s, 3),
+            "training": {"samples": int(samples), "sources": sources, "adam_steps": 4000,
+                         "batch": 16384, "seconds": round(train_s, 3),
                          "mse_first_last": [float("%.3e" % losses[0]), float("%.3e" % losses[-1])]}}
+
+
+def accuracy_leg():
+    """The FP64 path against the oracle at the stratified 128^3
+    configuration: 200 RK4 steps with the oracle's dt sequence, compared on
+    the cells the committed oracle fixture holds (every 8th cell along each
+    axis and the plane at axis-3 index 64).  Per-variable rel-L2 and max-abs,
+    beside the oracle's own FMA/no-FMA spread; None without the fixture."""
+    import numpy as np
+    from sac_amd import device, problems
+    from sac_amd.errors import mesh
+    path = os.path.join(ROOT, "tests", "golden", "stratified_128_200steps.npz")
+    if not os.path.exists(path):
+        return None
+    fx = np.load(path)
+    prob = problems.stratified(n=128)
+    dev = device.Device(prob)
+    dev.upload(prob.w)
+    dev.getboundary()
+    for dt in fx["dts"]:
+        dev.advance(float(dt))
+    w = dev.download()
+    dev.close()
+    m = mesh(prob, w[:8])
+    got = np.concatenate([m[:, ::8, ::8, ::8].reshape(8, -1), m[:, 64].reshape(8, -1)], axis=1)
+    ref = np.concatenate([fx["sub"].reshape(8, -1), fx["slice_k"].reshape(8, -1)], axis=1)
+    rel = [float(np.linalg.norm(got[i] - ref[i]) / np.linalg.norm(ref[i])) for i in range(8)]
+    mx = [float(np.abs(got[i] - ref[i]).max()) for i in range(8)]
+    return {"config": "stratified 128^3, 200 RK4 steps, FP64, the oracle's dt sequence; "
+                      "cells: every 8th along each axis + the plane at axis-3 index 64",
+            "rel_l2": [float("%.3e" % e) for e in rel],
+            "max_abs": [float("%.3e" % e) for e in mx],
+            "oracle_fma_spread": [float("%.3e" % e) for e in fx["floor"]]}
 
 
 def measured_traffic(precision, n):

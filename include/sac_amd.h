@@ -72,6 +72,7 @@ extern "C" {
 #define SAC_K_AXIS2 6
 #define SAC_K_AXIS3 7
 #define SAC_K_CLOSURE 8 /* learned-closure inference launches (one per axis) */
+#define SAC_K_PLANE 9   /* 3-D grids: the x and y work of z-plane tiles (one launch per stage) */
 
 typedef void *sac_handle;
 
@@ -167,6 +168,16 @@ int sac_dev_getdt_special(sac_handle h, double *dt);
  * one full Runge-Kutta step of the state including every getboundary. */
 int sac_dev_advance(sac_handle h, double dt);
 
+/* Kernel schedule of the stages of 3-D grids: with on=1 (the default) the x
+ * and y work of a stage runs in one pass over 2-D tiles of each z-plane
+ * (the x-y cross terms included), then the z line tiles; with on=0 one
+ * line-tile pass per axis.  Both evaluate the same terms of centdiff4,
+ * setnu and addsource_visc (src/vaccd.t:7-78, src/vacusr.viscosity.t:2-455)
+ * and agree to rounding (the cross terms are summed in another order).  The
+ * learned closure and Powell's sources use the per-axis passes.  No
+ * counterpart in the reference (a schedule choice of this library). */
+int sac_dev_set_plane_tiles(sac_handle h, int on);
+
 /* divbfix: Powell's div B source terms (addsource_divb,
  * src/vacphys.mhd0.t:256-297) added to every advect1 when on and ndim > 1
  * (the reference adds nothing in 1-D).  Off after sac_dev_init. */
@@ -226,6 +237,10 @@ int sac_closure_get_params(sac_handle h, float *p, int n);
 /* Append (stencil, exact ratio) samples of every mesh face of every axis of
  * register `slot`; faces past the sensor threshold get weight 0. */
 int sac_closure_collect(sac_handle h, int slot, long long *total);
+/* The same samples from register `slot` of another handle `src` (any grid
+ * and dimensionality, FP64 storage, same device) into h's buffer: a
+ * network trained on one configuration's samples from the runs of others. */
+int sac_closure_collect_from(sac_handle h, sac_handle src, int slot, long long *total);
 int sac_closure_clear(sac_handle h);
 /* `steps` Adam steps of blocks*64 samples each; sample k of the run is
  * ((base + k) * stride) mod n (stride coprime with n walks a permutation).

@@ -17,7 +17,8 @@ from .device import SLOT_W, SacError, load_library
 
 EXPORTS = [
     "sac_closure_nparam", "sac_closure_init", "sac_closure_set_params",
-    "sac_closure_get_params", "sac_closure_collect", "sac_closure_clear",
+    "sac_closure_get_params", "sac_closure_collect", "sac_closure_collect_from",
+    "sac_closure_clear",
     "sac_closure_train", "sac_closure_grad", "sac_closure_enable",
 ]
 
@@ -39,6 +40,7 @@ def _lib():
             "sac_closure_set_params": (I, [V, fp, I]),
             "sac_closure_get_params": (I, [V, fp, I]),
             "sac_closure_collect": (I, [V, I, ctypes.POINTER(LL)]),
+            "sac_closure_collect_from": (I, [V, V, I, ctypes.POINTER(LL)]),
             "sac_closure_clear": (I, [V]),
             "sac_closure_train": (I, [V, I, I, D, LL, LL, ctypes.POINTER(D)]),
             "sac_closure_grad": (I, [V, fp, fp, I, fp, ctypes.POINTER(D)]),
@@ -88,6 +90,13 @@ class Closure:
     def collect(self, slot=SLOT_W):
         n = ctypes.c_longlong()
         _check(_lib().sac_closure_collect(self.dev.handle, slot, ctypes.byref(n)))
+        self.n = n.value
+        return self.n
+
+    def collect_from(self, src, slot=SLOT_W):
+        """Samples of another Device's state (FP64, same GPU) into this buffer."""
+        n = ctypes.c_longlong()
+        _check(_lib().sac_closure_collect_from(self.dev.handle, src.handle, slot, ctypes.byref(n)))
         self.n = n.value
         return self.n
 

@@ -109,8 +109,11 @@ struct Dev {
   long long bytes = 0;
   bool timing = false;
   std::vector<TimedLaunch> timed;
-  double t_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  long long t_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double t_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long long t_n[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // 3-D stages: the x and y work in one pass over z-plane tiles (k_plane)
+  // instead of one line-tile pass per axis
+  int plane = 1;
   int xcd = 8;  // XCDs of the device (block renumbering of the line kernels)
   Closure* cl = nullptr;
   // asynchronous saves (sac_dev_save_*): a device copy of W taken on the
@@ -225,7 +228,20 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   x.red_visc = red_visc(d);
   const bool learned = d.cl && d.cl->on;
   int skip_last = 0;
-  for (int k = 0; k < d.ndim; k++) {
+  // plane tiles: the x and y work (and their cross terms) in one launch,
+  // then the z line tiles (which add the z cross terms), then k_final
+  const bool plane = d.plane && d.ndim == 3 && !learned && !d.divbfix && sac::line_tiles_built();
+  if (plane) {
+    x.xcd = d.xcd;
+    x.RF = nullptr;
+    x.cross_last = 0;
+    x.divbfix = 0;
+    hipEvent_t e0{};
+    t_begin(d, SAC_K_PLANE, &e0);
+    sac::launch_plane(x, d.stream);
+    t_end(d, SAC_K_PLANE, e0);
+  }
+  for (int k = plane ? 2 : 0; k < d.ndim; k++) {
     // axis 1 (rows of a plane): neighbour rows in one XCD's L2.  Axes 0 and 2
     // keep the hardware order, whose round-robin already places the
     // neighbour planes of a row in the same XCD.
@@ -284,6 +300,7 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   s.red_out = red_of(d, out);
   s.xcd = 1;
   s.skip_last = skip_last;
+  s.skip_xy = plane ? 1 : 0;
   hipEvent_t e0{};
   t_begin(d, SAC_K_FINAL, &e0);
   sac::launch_final(s, d.stream);
@@ -865,6 +882,13 @@ int sac_dev_set_resistive(sac_handle h, int compactres, double dtdiffpar) {
   return 0;
 }
 
+int sac_dev_set_plane_tiles(sac_handle h, int on) {
+  if (int rc = check_handle(h)) return rc;
+  Dev& d = *(Dev*)h;
+  d.plane = on ? 1 : 0;
+  return 0;
+}
+
 int sac_dev_set_divbfix(sac_handle h, int on) {
   if (int rc = check_handle(h)) return rc;
   Dev& d = *(Dev*)h;
@@ -930,7 +954,7 @@ int sac_dev_timing(sac_handle h, int enable) {
   Dev& d = *(Dev*)h;
   hipStreamSynchronize(d.stream);
   t_collect(d);
-  for (int k = 0; k < 9; k++) {
+  for (int k = 0; k < 10; k++) {
     d.t_ms[k] = 0.0;
     d.t_n[k] = 0;
   }
@@ -940,7 +964,7 @@ int sac_dev_timing(sac_handle h, int enable) {
 
 int sac_dev_timing_get(sac_handle h, int kernel, double* total_ms, long long* count) {
   if (int rc = check_handle(h)) return rc;
-  if (kernel < 0 || kernel > 8) return fail(SAC_E_ARG, "kernel id %d", kernel);
+  if (kernel < 0 || kernel > 9) return fail(SAC_E_ARG, "kernel id %d", kernel);
   Dev& d = *(Dev*)h;
   t_collect(d);
   if (total_ms) *total_ms = d.t_ms[kernel];
@@ -1063,6 +1087,30 @@ int sac_closure_collect(sac_handle h, int slot, long long* total) {
   if (c->n + add > c->cap)
     return fail(SAC_E_ARG, "sample buffer full (%lld + %lld > %lld)", c->n, add, c->cap);
   sac::closure::launch_collect(d.ndim, d.ndir, d.g, (const double*)d.reg[slot], d.bg, d.gamma,
+                               c->sensor, c->feat, c->targ, c->n, d.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d.stream));
+  c->n += add;
+  if (total) *total = c->n;
+  return 0;
+}
+
+int sac_closure_collect_from(sac_handle h, sac_handle src, int slot, long long* total) {
+  Closure* c;
+  if (int rc = closure_of(h, &c)) return rc;
+  if (int rc = check_handle(src)) return rc;
+  Dev& d = *(Dev*)h;
+  Dev& s = *(Dev*)src;
+  if (slot < 0 || slot >= kSlots) return fail(SAC_E_ARG, "bad slot %d", slot);
+  if (s.prec != SAC_PREC_FP64) return fail(SAC_E_UNSUPPORTED, "samples come from FP64 storage");
+  if (s.device != d.device) return fail(SAC_E_ARG, "the two handles are on different devices");
+  long long nm = 1;
+  for (int k = 0; k < 3; k++) nm *= s.g.Mhi[k] - s.g.Mlo[k] + 1;
+  const long long add = nm * s.ndim;
+  if (c->n + add > c->cap)
+    return fail(SAC_E_ARG, "sample buffer full (%lld + %lld > %lld)", c->n, add, c->cap);
+  HIPCHK(hipStreamSynchronize(s.stream));  // the source state is complete
+  sac::closure::launch_collect(s.ndim, s.ndir, s.g, (const double*)s.reg[slot], s.bg, s.gamma,
                                c->sensor, c->feat, c->targ, c->n, d.stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(d.stream));

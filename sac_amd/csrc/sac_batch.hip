@@ -139,8 +139,7 @@ __device__ __forceinline__ bool tile_cell(const Args& a, const Tile& tl, const G
 // of a patch also clears the patch's reductions of register `out`.
 template <int K>
 __global__ void __launch_bounds__((line::Tile<K, double>::T)) kb_pline(Args a, int wct, int out) {
-  __shared__ double L[line::Slots<2, NDIR>::NV * line::Tile<K, double>::RR *
-                      line::Tile<K, double>::W];
+  __shared__ double L[line::lds_elems<2, NDIR, K, double, false>()];
   const Tile tl = a.ltiles[K][blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
   if (K == 0 && threadIdx.x < kRed && tl.c0 == 0) red_of(a, tl.tube, out)[threadIdx.x] = 0ull;

@@ -723,6 +723,7 @@ __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g
 #pragma unroll
       for (int k = 0; k < NDIM; k++) {
         if (k == l) continue;
+        if (a.skip_xy && k + l == 1) continue;  // added by the plane tiles
         const long long xi = (long long)cross_index(NDIM, k, l) * fs;
         const C Tmp0 = (C)TT[xi + ip], Tmm0 = (C)TT[xi + im];
         const C Tbp0 = (C)TT[xi + fs + ip], Tbm0 = (C)TT[xi + fs + im];

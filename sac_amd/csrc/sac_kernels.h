@@ -45,6 +45,7 @@ struct FinalArgs {
   unsigned long long* red_out;  // cmax reductions of out
   int xcd;
   int skip_last;                // cross-term derivatives along the last axis already in R
+  int skip_xy;                  // the x-y cross-term derivatives already in R (plane tiles)
 };
 
 struct CmaxArgs {
@@ -87,6 +88,9 @@ void launch_resistive(const ResistArgs& a, hipStream_t st);
 int launch_axis(const AxisArgs& a, int axis, hipStream_t st);
 // 1 if the line-tile kernels (which carry the Powell sources) are built
 int line_tiles_built();
+// the x and y work of a 3-D grid in one pass over z-plane tiles (sac_plane.h),
+// the x-y cross terms included; R is written fresh
+void launch_plane(const AxisArgs& a, hipStream_t st);
 void launch_final(const FinalArgs& a, hipStream_t st);
 void launch_cmax(const CmaxArgs& a, hipStream_t st);
 void launch_ghost(const GhostArgs& a, hipStream_t st);

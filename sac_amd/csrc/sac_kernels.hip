@@ -25,6 +25,7 @@
 //   getboundary        src/vacgrid.t:195-512
 #include "sac_device.h"
 #include "sac_line.h"
+#include "sac_plane.h"
 
 // SAC_LINE_TILES=0 keeps the line-group kernel k_axis for every grid (the
 // default runs 2-D and 3-D grids through the LDS line tiles, k_line)
@@ -71,11 +72,24 @@ __global__ void __launch_bounds__(256) k_axis(AxisArgs a) {
 }
 
 // kernel 1, 2-D and 3-D grids: LDS-staged line tiles of axis K (see line_tile)
-template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
+template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false,
+          bool DIVB = false>
 __global__ void __launch_bounds__((line::Tile<K, C>::T)) k_line(AxisArgs a, int seg) {
-  __shared__ C L[line::Slots<NDIM, NDIR>::NV * line::Tile<K, C>::RR * line::Tile<K, C>::W];
+  __shared__ C L[line::lds_elems<NDIM, NDIR, K, C, DIVB>()];
   C numax = 0;
-  line::line_tile<NDIM, NDIR, K, S, C, LEARNED>(a, seg, L, numax);
+  line::line_tile<NDIM, NDIR, K, S, C, LEARNED, DIVB>(a, seg, L, numax);
+  if (a.visc_on) {
+    double r2[1] = {(double)numax};
+    block_max_atomic<1>(r2, a.red_visc);
+  }
+}
+
+// kernel 1, 3-D grids: the x and y work of z-plane tiles (see plane_tile)
+template <typename S, typename C>
+__global__ void __launch_bounds__(plane::NT) k_plane(AxisArgs a) {
+  __shared__ plane::Smem<C> sm;
+  C numax = 0;
+  plane::plane_tile<S, C>(a, sm, numax);
   if (a.visc_on) {
     double r2[1] = {(double)numax};
     block_max_atomic<1>(r2, a.red_visc);
@@ -112,10 +126,14 @@ static void launch_line(const AxisArgs& A, hipStream_t st) {
   const int seg = line_segment<NDIM, K, C>(A.g);
   long long nb = line::nblocks<NDIM, K, C>(A.g, seg);
   nb = (nb + 15) / 16 * 16;  // tile pairing over the XCDs (see line_tile)
-  if (nb > 0)
-    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C, LEARNED>), dim3((unsigned)nb),
-                       dim3(line::Tile<K, C>::T),
-                       0, st, A, seg);
+  if (nb <= 0) return;
+  // Powell's sources need one more ring value per point: a kernel variant
+  if (A.divbfix)
+    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C, LEARNED, true>), dim3((unsigned)nb),
+                       dim3(line::Tile<K, C>::T), 0, st, A, seg);
+  else
+    hipLaunchKernelGGL((k_line<NDIM, NDIR, K, S, C, LEARNED, false>), dim3((unsigned)nb),
+                       dim3(line::Tile<K, C>::T), 0, st, A, seg);
 }
 
 // kernel 2: the combining update of every mesh cell (see final_cell)
@@ -366,6 +384,17 @@ int launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
   dispatch(a.prec, g.ndim, g.ndir, 0, axis, &a, st, n);
   const bool line = SAC_LINE_TILES && g.ndim > 1;
   return (line && a.cross_last && a.visc_on && axis == g.ndim - 1) ? 1 : 0;
+}
+
+void launch_plane(const AxisArgs& a, hipStream_t st) {
+  long long nb = plane::nblocks(a.g);
+  const int xcd = a.xcd > 1 ? a.xcd : 1;
+  nb = (nb + xcd - 1) / xcd * xcd;  // xcd_block needs a multiple of the XCD count
+  if (nb <= 0) return;
+  if (a.prec == PREC_MIXED)
+    hipLaunchKernelGGL((k_plane<float, float>), dim3((unsigned)nb), dim3(plane::NT), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_plane<double, double>), dim3((unsigned)nb), dim3(plane::NT), 0, st, a);
 }
 
 void launch_final(const FinalArgs& a, hipStream_t st) {

@@ -12,10 +12,10 @@
 // along K in chunks of CH points (16 x 16 along y and z; along x 4 rows of
 // 64 points in FP64, 2 rows of 128 in FP32 storage; see Tile).  Every
 // thread loads ONE new point per chunk (13 fields, coalesced along x),
-// evaluates everything that depends on that point alone (the flux and
-// velocities along K, p_Tb and the background field, rho_t, momenta and
-// the seven setnu q values) once, and stores it in a ring of CH+6 rows in
-// LDS.  After a barrier the thread
+// evaluates everything that depends on that point alone (the flux along K,
+// rho_t, momenta and the seven setnu q values, which include the
+// velocities) once, and stores it in a ring of CH+6 rows in LDS.  After a
+// barrier the thread
 // assembles the cell three points behind its point from the ring: the
 // flux differences of points c-2..c+2 and the face ratios of the 7-point q
 // window c-3..c+3.  Each point is loaded from HBM and evaluated once per
@@ -53,20 +53,31 @@ struct Tile {
   static constexpr int RR = CH + 2 * HALO;  // ring rows
 };
 
-template <int NDIM, int NDIR>
+// Values per point in the ring.  Only values read at a neighbouring point
+// are kept: the background terms of a cell (p_Tb, bg) are read from memory by
+// the cell's own thread, and v_l = m_l/rho_t is setnu's q of v_l.  At 19
+// FP64 values per point (27 before) three x tiles fit in a CU's LDS instead
+// of two (the x kernel 14 % faster at 256^3); the y and z tiles stay at two
+// blocks, set by their registers.
+template <int NDIM, int NDIR, bool DIVB = false>
 struct Slots {
   static constexpr int NE = 2 + 2 * NDIR;
   static constexpr int NP = 2 * NDIM + 1;
   static constexpr int F0 = 0;          // flux along K of the NE evolving fields
-  static constexpr int V0 = F0 + NE;    // v_l, l < NDIM
-  static constexpr int PB = V0 + NDIM;  // p_Tb (getptotal_bg)
-  static constexpr int B0 = PB + 1;     // bg_l, l < NDIM
-  static constexpr int RT = B0 + NDIM;  // rho_t
+  static constexpr int RT = F0 + NE;    // rho_t
   static constexpr int M0 = RT + 1;     // m_l, l < NDIM
-  static constexpr int Q0 = M0 + NDIM;  // setnu q of the NP variables
+  static constexpr int Q0 = M0 + NDIM;  // setnu q of the NP variables: rho', thermal
+                                        // energy, v_l (l < NDIM), b_l (l != K)
+  static constexpr int V0 = Q0 + 2;     // v_l, l < NDIM (the q of v_l)
   static constexpr int BT = Q0 + NP;    // b_K + bg_K (div B of Powell's sources)
-  static constexpr int NV = BT + 1;
+  static constexpr int NV = BT + (DIVB ? 1 : 0);
 };
+
+// LDS elements of a line tile (the ring)
+template <int NDIM, int NDIR, int K, typename C, bool DIVB>
+constexpr int lds_elems() {
+  return Slots<NDIM, NDIR, DIVB>::NV * Tile<K, C>::RR * Tile<K, C>::W;
+}
 
 template <int K, typename C>
 __device__ __forceinline__ int at(int v, int row, int col) {
@@ -106,10 +117,11 @@ __host__ __device__ inline long long nblocks(const GridDesc& g, int seg) {
 // stored at the face's left cell; < 0 where the exact ratio is required).
 // blk >= 0 gives the block's logical index directly (patch ensembles, one
 // grid per patch); otherwise it follows from blockIdx.x.
-template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false>
+template <int NDIM, int NDIR, int K, typename S, typename C, bool LEARNED = false,
+          bool DIVB = false>
 __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restrict__ L,
                                           C& numax, long long blk = -1) {
-  using SL = Slots<NDIM, NDIR>;
+  using SL = Slots<NDIM, NDIR, DIVB>;
   constexpr int NE = SL::NE, NP = SL::NP;
   constexpr int A = Axes<NDIM, K>::A, B = Axes<NDIM, K>::B;
   constexpr int CH = Tile<K, C>::CH, W = Tile<K, C>::W, RR = Tile<K, C>::RR;
@@ -217,6 +229,11 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
 #pragma unroll
     for (int n = 0; n < NE; n++) Rold[n] = (K > 0) ? (C)Racc[n * fs + icell] : (C)0;
     const double dxd = a.dx[K][cell_clamped(j)];
+    // the cell's background: p_Tb (getptotal_bg) and bg_l of the energy terms
+    Pt<NDIR, C> pbg;
+    pbg.eb = Bg[icell];
+#pragma unroll
+    for (int c = 0; c < NDIR; c++) pbg.bg[c] = Bg[(2 + c) * fs + icell];
     C Tc[NTC];
     if constexpr (kCross) {
       if (do_cross) {
@@ -252,15 +269,10 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
 #pragma unroll
       for (int n = 0; n < NE; n++) L[at<K, C>(SL::F0 + n, r0, iw)] = F[n];
 #pragma unroll
-      for (int l = 0; l < NDIM; l++) {
-        L[at<K, C>(SL::V0 + l, r0, iw)] = v[l];
-        L[at<K, C>(SL::B0 + l, r0, iw)] = (C)pt.bg[l];
-        L[at<K, C>(SL::M0 + l, r0, iw)] = pt.m[l];
-      }
-      L[at<K, C>(SL::PB, r0, iw)] = (C)ptotal_bg<NDIR, C>(pt, gamma);
+      for (int l = 0; l < NDIM; l++) L[at<K, C>(SL::M0 + l, r0, iw)] = pt.m[l];
       const double rtd = (double)pt.rho + pt.rhob;
       L[at<K, C>(SL::RT, r0, iw)] = (C)rtd;
-      L[at<K, C>(SL::BT, r0, iw)] = (C)((double)pt.b[K] + pt.bg[K]);
+      if constexpr (DIVB) L[at<K, C>(SL::BT, r0, iw)] = (C)((double)pt.b[K] + pt.bg[K]);
       {
 #pragma clang fp contract(off)
         // q of setnu: rho', thermal energy (in FP64), v_l, b_l (l != K)
@@ -297,7 +309,7 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
       C R[NE];
       {
         const bool fluxc = c >= MloK + 2 && c <= MhiK - 2;
-        const int rm2 = row(-2), rm1 = row(-1), rp1 = row(1), rp2 = row(2), r00 = row(0);
+        const int rm2 = row(-2), rm1 = row(-1), rp1 = row(1), rp2 = row(2);
         const C inv12 = idx / (C)12;
 #pragma unroll
         for (int n = 0; n < NE; n++) {
@@ -306,8 +318,8 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
           const C D2 = two_diff(L[at<K, C>(SL::F0 + n, rp2, iw)], L[at<K, C>(SL::F0 + n, rm2, iw)], e2);
           R[n] = ((D2 - (C)8 * D1) + (e2 - (C)8 * E1)) * inv12;
         }
-        const C pTb0 = L[at<K, C>(SL::PB, r00, iw)];
-        const C bgk = L[at<K, C>(SL::B0 + K, r00, iw)];
+        const C pTb0 = (C)ptotal_bg<NDIR, C>(pbg, gamma);
+        const C bgk = (C)pbg.bg[K];
         C re = 0;
 #pragma unroll
         for (int l = 0; l < NDIM; l++) {
@@ -315,7 +327,7 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
           const C dv2 = L[at<K, C>(SL::V0 + l, rp2, iw)] - L[at<K, C>(SL::V0 + l, rm2, iw)];
           const C Dv = -(dv2 - (C)8 * dv1) * inv12;
           if (l == K) re -= Dv * pTb0;
-          re += bgk * L[at<K, C>(SL::B0 + l, r00, iw)] * Dv;
+          re += bgk * (C)pbg.bg[l] * Dv;
         }
         R[1 + NDIR] += re;
 #pragma unroll
@@ -325,7 +337,7 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
         // outermost cells like the flux), so each axis adds its share
         //   m -= (b+bg) g_K,  b -= m/rho_t g_K,  e -= m.(b+bg)/rho_t g_K
         // with the state of the cell (loaded here: all ndir components)
-        if (a.divbfix && valid && fluxc) {
+        if (DIVB && valid && fluxc) {
           const C D1b = L[at<K, C>(SL::BT, rp1, iw)] - L[at<K, C>(SL::BT, rm1, iw)];
           const C D2b = L[at<K, C>(SL::BT, rp2, iw)] - L[at<K, C>(SL::BT, rm2, iw)];
           const C gK = -((D2b - (C)8 * D1b) * inv12);

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libsac_amd.so")
 SLOT_W, SLOT_S1, SLOT_S2, SLOT_ACC = 0, 1, 2, 3
 K_FINAL, K_AXIS, K_GHOST, K_CMAX, K_COPY = 0, 1, 2, 3, 4
 K_AXIS1, K_AXIS2, K_AXIS3 = 5, 6, 7
+K_CLOSURE, K_PLANE = 8, 9
 ADV = {"onestep": 1, "twostep": 2, "threestep": 3, "fourstep": 4, "sterck": 5, "jameson": 6}
 PREC = {"fp64": 0, "mixed": 1}
 
@@ -26,7 +27,7 @@ EXPORTS = [
     "sac_dev_init", "sac_dev_free", "sac_dev_upload", "sac_dev_download",
     "sac_dev_upload_slot", "sac_dev_download_slot", "sac_dev_getboundary",
     "sac_dev_getdt_courant", "sac_dev_getdt_special", "sac_dev_advance",
-    "sac_dev_advect1", "sac_dev_set_divbfix", "sac_dev_set_resistive", "sac_dev_copy_slot", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
+    "sac_dev_advect1", "sac_dev_set_divbfix", "sac_dev_set_plane_tiles", "sac_dev_set_resistive", "sac_dev_copy_slot", "sac_dev_get_maxviscoef", "sac_dev_set_maxviscoef",
     "sac_dev_timing", "sac_dev_timing_get", "sac_dev_synchronize",
     "sac_dev_memory", "sac_dev_save_begin", "sac_dev_save_ready", "sac_dev_save_end",
     "sac_getdt_courant_noalloc", "sac_dev_last_error",
@@ -65,6 +66,7 @@ def load_library(path: str = LIB_PATH):
         "sac_dev_advance": (I, [V, D]),
         "sac_dev_advect1": (I, [V, D, I, I]),
         "sac_dev_set_divbfix": (I, [V, I]),
+        "sac_dev_set_plane_tiles": (I, [V, I]),
         "sac_dev_set_resistive": (I, [V, I, D]),
         "sac_dev_copy_slot": (I, [V, I, I]),
         "sac_dev_get_maxviscoef": (I, [V, dp]),
@@ -224,6 +226,11 @@ class Device:
     def set_resistive(self, compactres=False, dtdiffpar=0.5):
         """Resistive source form and dtdiffpar (eta = eqpar[1] > 0)."""
         _check(load_library().sac_dev_set_resistive(self._h, 1 if compactres else 0, dtdiffpar))
+
+    def set_plane_tiles(self, on=True):
+        """3-D stages: x and y work in one pass over z-plane tiles (default)
+        or one line-tile pass per axis."""
+        _check(load_library().sac_dev_set_plane_tiles(self._h, 1 if on else 0))
 
     def set_divbfix(self, on=True):
         """Powell's div B sources in every advect1 (ndim > 1)."""

@@ -138,3 +138,55 @@ def test_learned_closure_gain_over_constant(libs, trained):
     # measured on MI355X: mean 0.083 (learned) against 0.19 (best constant)
     assert np.mean(learned) < 0.6 * np.mean(best)
     assert max(learned) < max(best)
+
+
+def test_learned_closure_targets(libs, trained):
+    """What the network gains over the plain method, measured against its
+    targets: at least 2.3x the speed of the plain FP64 path and rel-L2 <=
+    1e-5 against it.  12 steps of the stratified setup at amplitude 0.1 on
+    96^3, both runs under the same dt sequence, the network of the fixture.
+    Unmet targets are reported as an expected failure with the measured
+    speed ratio and error (they are not reached: the exact ratio costs two
+    face-ratio evaluations on data already in LDS, the network a pass of its
+    own with BF16 weights)."""
+    import time
+    device, closure = libs
+    P = trained[3]
+    prob = problems.stratified(n=96, amp=0.1)
+    dev = device.Device(prob)
+    dev.upload(prob.w)
+    dev.getboundary()
+    dts = [dev.step(prob.courantpar) for _ in range(12)]
+    dev.close()
+
+    def run(learned):
+        d = device.Device(prob)
+        try:
+            d.upload(prob.w)
+            d.getboundary()
+            if learned:
+                c = closure.Closure(d, capacity=0, max_blocks=8, seed=1)
+                c.params = P
+                c.enable(True, 0.1)
+            d.advance(dts[0])
+            d.upload(prob.w)
+            d.getboundary()
+            d.synchronize()
+            t0 = time.perf_counter()
+            for dt in dts:
+                d.advance(dt)
+            d.synchronize()
+            return time.perf_counter() - t0, d.download()
+        finally:
+            d.close()
+
+    t_plain, w_plain = run(False)
+    t_net, w_net = run(True)
+    err = per_var_errors(prob, w_net, w_plain, 8)
+    speed = t_plain / t_net
+    print(f"learned closure at 96^3: {speed:.3f}x the plain FP64 speed, rel-L2 " +
+          " ".join(f"{e:.2e}" for e in err))
+    assert all(np.isfinite(err))
+    if speed < 2.3 or max(err) > 1e-5:
+        pytest.xfail(f"targets not met: {speed:.2f}x (target 2.3x), rel-L2 up to "
+                     f"{max(err):.2e} (target 1e-5)")
