@@ -108,7 +108,8 @@ int sac_dev_free(sac_handle h);
 
 /* Copy the state w (reference layout, nw variables, whole grid) to / from
  * the device.  The static background (variables eb..bg) is taken from the
- * upload and returned unchanged by the download. */
+ * upload and returned unchanged by the download; the ghost cells of an
+ * uploaded background are filled again by the next sac_dev_getboundary. */
 int sac_dev_upload(sac_handle h, const double *w);
 int sac_dev_download(sac_handle h, double *w);
 
@@ -132,7 +133,9 @@ int sac_dev_upload_slot(sac_handle h, int slot, const double *w);
 int sac_dev_download_slot(sac_handle h, int slot, double *w);
 
 /* getboundary(t,1,nw,1,ndim,w) on the state (src/vacgrid.t:195-512); the
- * first call stores the 'fixed' boundary values as the reference does. */
+ * first call fills the background's ghosts and stores the 'fixed' boundary
+ * values as the reference does (later calls fill the background's ghosts
+ * only after an upload). */
 int sac_dev_getboundary(sac_handle h);
 
 /* getdt_courant (src/vac.t:641-699): dt = courantpar/max(cmax_d/dx_d) over

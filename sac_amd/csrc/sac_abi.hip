@@ -94,6 +94,9 @@ struct Dev {
   // their values are stored (evolving fields, background)
   bool cont1_e = false, ext1_e = false, cont1_b = false, ext1_b = false;
   bool initialized = false;  // getboundary's 'initialized' (src/vacgrid.t:209)
+  // an upload after the first getboundary replaced the background's ghost
+  // cells with the caller's: the next getboundary fills them again
+  bool bg_ghosts_stale = false;
   // device buffers (evolving fields, R, T and snap in the storage precision)
   void* reg[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   double* bg = nullptr;
@@ -624,6 +627,7 @@ int sac_dev_upload(sac_handle h, const double* w) {
   if (int rc = copy_in(d, d.bg, w + (size_t)d.ne * d.ncell_grid, d.nbg, sizeof(double))) return rc;
   d.red_valid[SAC_SLOT_W] = false;
   d.bg_host.clear();  // the ghosts of the background change at the next getboundary
+  d.bg_ghosts_stale = d.initialized;
   return 0;
 }
 
@@ -725,8 +729,9 @@ int sac_dev_save_end(sac_handle h, double* w) {
 int sac_dev_getboundary(sac_handle h) {
   if (int rc = check_handle(h)) return rc;
   Dev& d = *(Dev*)h;
-  if (!d.initialized) {
-    // first call: background ghosts too, then store 'fixed' values
+  if (!d.initialized || d.bg_ghosts_stale) {
+    // background ghosts: on the first call (the background is static, so
+    // the reference fills them once) and after an upload replaced them
     sac::GhostArgs a{};
     a.g = d.g;
     a.prec = sac::PREC_FP64;
@@ -737,10 +742,15 @@ int sac_dev_getboundary(sac_handle h) {
     a.keep_fixed = d.fullgridini ? 1 : 0;
     a.general = d.cont1_b || (d.ext1_b && !a.keep_fixed);
     sac::launch_ghost(a, d.stream);
+    d.bg_ghosts_stale = false;
+    d.bg_host.clear();  // background ghosts filled: the cached copy is stale
+  }
+  if (!d.initialized) {
+    // first call: then store 'fixed' values
     ghost_fill_evolving(d, SAC_SLOT_W);
     if (d.any_fixed) {
       // fixed/fixed1 values and grad1 differences of the first fill
-      // (src/vacgrid.t:473-507); the background is filled only once
+      // (src/vacgrid.t:473-507)
       sac::GhostArgs st{};
       st.g = d.g;
       st.prec = d.prec;
@@ -750,7 +760,6 @@ int sac_dev_getboundary(sac_handle h) {
       sac::launch_ghost_store(st, d.snap, d.stream);
     }
     d.initialized = true;
-    d.bg_host.clear();  // background ghosts filled: the cached copy is stale
   } else {
     ghost_fill_evolving(d, SAC_SLOT_W);
   }

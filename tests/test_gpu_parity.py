@@ -67,6 +67,34 @@ def test_getboundary_all_types(dev_mod, oracle_mod):
         dev.close()
 
 
+def test_upload_after_first_fill_refills_background_ghosts(dev_mod):
+    """An upload after the first getboundary brings the caller's background
+    ghost cells (zero in a fresh problem array); the next getboundary fills
+    them again, so the run equals one on a fresh handle bit for bit."""
+    prob = problems.stratified(n=24, nxyz=(24, 20, 12), amp=0.1)
+    dts = [2e-4, 1e-4, 1e-4]
+    dev = dev_mod.Device(prob)
+    dev.upload(prob.w)
+    dev.getboundary()
+    for dt in dts:
+        dev.advance(dt)
+    ref = dev.download()
+    dev.close()
+    dev = dev_mod.Device(prob)
+    dev.upload(prob.w)
+    dev.getboundary()
+    dev.advance(dts[0])
+    dev.advance(dts[1])
+    dev.upload(prob.w)
+    dev.getboundary()
+    for dt in dts:
+        dev.advance(dt)
+    got = dev.download()
+    dev.close()
+    assert np.isfinite(got).all()
+    np.testing.assert_array_equal(got, ref)
+
+
 @pytest.mark.parametrize("case", ["briowu", "riemann3", "strat3d", "strat2d"])
 def test_advect1_stage(dev_mod, oracle_mod, case):
     """one advect1 ('cd4' + sources + getboundary) from an arbitrary wCT"""
