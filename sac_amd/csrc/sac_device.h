@@ -262,6 +262,27 @@ __device__ __forceinline__ C face_ratio(const C* q) {
   return (md1 > (C)0) ? cdiv(md3, md1) : (C)0;
 }
 
+// The ratios of both faces of a cell, c-1/2 and c+1/2, from its 7-point
+// window q[0..6] (c-3..c+3): equal to face_ratio(q) and face_ratio(q + 1).
+// The two windows share four of their five first differences and two of
+// their three third differences; the maximum over the shared ones is taken
+// once (max is exact, so the grouping does not change the result).
+template <typename C>
+__device__ __forceinline__ void face_pair(const C* q, C& ratL, C& ratR) {
+#pragma clang fp contract(off)
+  C s[6], d3[4];
+#pragma unroll
+  for (int j = 0; j < 6; j++) s[j] = q[j + 1] - q[j];
+#pragma unroll
+  for (int j = 0; j < 4; j++) d3[j] = fabs((C)3 * s[j + 1] - (q[j + 3] - q[j]));
+  const C m1 = fmax(fmax(fabs(s[1]), fabs(s[2])), fmax(fabs(s[3]), fabs(s[4])));
+  const C m3 = fmax(d3[1], d3[2]);
+  const C md1L = fmax(m1, fabs(s[0])), md1R = fmax(m1, fabs(s[5]));
+  const C md3L = fmax(d3[0], m3), md3R = fmax(m3, d3[3]);
+  ratL = (md1L > (C)0) ? cdiv(md3L, md1L) : (C)0;
+  ratR = (md1R > (C)0) ? cdiv(md3R, md1R) : (C)0;
+}
+
 __device__ __forceinline__ int cross_index(int ndim, int k, int l) {
   return 2 * (k * (ndim - 1) + (l < k ? l : l - 1));
 }
@@ -636,7 +657,8 @@ __device__ __forceinline__ void line1_tile(const GridDesc& g, const double* __re
       for (int s2 = 0; s2 < 7; s2++) q[s2] = at(NE + 5 + pq, k - 3 + s2);
       const double ch = (pq == 2) ? 0.4 : 0.04;
       const double Kc = ctk * ch;
-      const double ratio0 = face_ratio<double>(q), ratio1 = face_ratio<double>(q + 1);
+      double ratio0, ratio1;
+      face_pair<double>(q, ratio0, ratio1);
       const double nuR = Kc * ratio1 * dxk;
       const double nuL = Kc * ratio0 * dxk;
       numax = valid ? fmax(numax, fmax(nuR, nuL)) : numax;

@@ -400,8 +400,7 @@ __device__ __forceinline__ void line_tile(const AxisArgs& a, int seg, C* __restr
             ratL = (rfc[2 * p] >= 0.f) ? (C)rfc[2 * p] : face_ratio<C>(q);
             ratR = (rfc[2 * p + 1] >= 0.f) ? (C)rfc[2 * p + 1] : face_ratio<C>(q + 1);
           } else {
-            ratL = face_ratio<C>(q);
-            ratR = face_ratio<C>(q + 1);
+            face_pair<C>(q, ratL, ratR);
           }
           const C nuL = Kc * ratL * dxk;
           const C nuR = Kc * ratR * dxk;

@@ -129,7 +129,8 @@ __device__ __forceinline__ void cell_axis(C (&R)[8], const Pt<3, C>& pbg, double
       C q[7];
 #pragma unroll
       for (int s = 0; s < 7; s++) q[s] = qget(qi, s - 3);
-      const C ratL = face_ratio<C>(q), ratR = face_ratio<C>(q + 1);
+      C ratL, ratR;
+      face_pair<C>(q, ratL, ratR);
       const C ch = (p >= 2 && p < 5) ? (C)0.4 : (C)0.04;
       const C Kc = ctk * ch;
       const C nuL = Kc * ratL * dxk;
@@ -183,8 +184,10 @@ __device__ __forceinline__ void cell_cross_factor(int l, int c, int MloK, int Mh
 #pragma unroll
     for (int s = 0; s < 7; s++) q[s] = qget(qi, s - 3);
     const C Kc = ctk * ((which == 0) ? (C)0.4 : (C)0.04);
-    const C nuL = Kc * face_ratio<C>(q) * dxk;
-    const C nuR = Kc * face_ratio<C>(q + 1) * dxk;
+    C ratL, ratR;
+    face_pair<C>(q, ratL, ratR);
+    const C nuL = Kc * ratL * dxk;
+    const C nuR = Kc * ratR * dxk;
     const C gC = z1 ? (C)0 : (q[4] - q[2]) * idx * (C)0.5;
     if (which == 0)
       Tm = gC * (nuL + nuR) * (C)0.25;
