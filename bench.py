@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-closure form (train + run at amplitude 0.1)")
     ap.add_argument("--learned-steps", type=int, default=12)
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the 128^3 leg against the oracle fixture")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run the library's plain copy kernel on one register "
                          "(known bytes) for HBM-counter calibration")
@@ -210,7 +212,7 @@ def main():
         out.setdefault("forms", {})["batched_patches_2d"] = patches_leg(args.patches)
     if not args.no_learned:
         out.setdefault("forms", {})["learned_closure"] = learned_leg(n, args.learned_steps)
-    acc = accuracy_leg()
+    acc = None if args.no_accuracy else accuracy_leg()
     if acc is not None:
         out["accuracy_vs_oracle"] = acc
     if not args.no_cpu_baseline:
@@ -404,8 +406,12 @@ def accuracy_leg():
     """The FP64 path against the oracle at the stratified 128^3
     configuration: 200 RK4 steps with the oracle's dt sequence, compared on
     the cells the committed oracle fixture holds (every 8th cell along each
-    axis and the plane at axis-3 index 64).  Per-variable rel-L2 and max-abs,
-    beside the oracle's own FMA/no-FMA spread; None without the fixture."""
+    axis, and the plane at axis-3 index 64).  Per-variable rel-L2 and
+    max-abs of each, beside the spread of the oracle itself: its FMA build
+    against its plain build, and its run from the initial state moved by one
+    ulp against its run from the exact one (how far apart two FP64 runs of
+    this configuration end when they differ by rounding alone); None without
+    the fixture."""
     import numpy as np
     from sac_amd import device, problems
     from sac_amd.errors import mesh
@@ -422,15 +428,19 @@ def accuracy_leg():
     w = dev.download()
     dev.close()
     m = mesh(prob, w[:8])
-    got = np.concatenate([m[:, ::8, ::8, ::8].reshape(8, -1), m[:, 64].reshape(8, -1)], axis=1)
-    ref = np.concatenate([fx["sub"].reshape(8, -1), fx["slice_k"].reshape(8, -1)], axis=1)
-    rel = [float(np.linalg.norm(got[i] - ref[i]) / np.linalg.norm(ref[i])) for i in range(8)]
-    mx = [float(np.abs(got[i] - ref[i]).max()) for i in range(8)]
-    return {"config": "stratified 128^3, 200 RK4 steps, FP64, the oracle's dt sequence; "
-                      "cells: every 8th along each axis + the plane at axis-3 index 64",
-            "rel_l2": [float("%.3e" % e) for e in rel],
-            "max_abs": [float("%.3e" % e) for e in mx],
-            "oracle_fma_spread": [float("%.3e" % e) for e in fx["floor"]]}
+    fmt = lambda v: [float("%.3e" % e) for e in v]  # noqa: E731
+    out = {"config": "stratified 128^3, 200 RK4 steps, FP64, the oracle's dt sequence",
+           "oracle_fma_spread": fmt(fx["floor"])}
+    for key, got in (("sub", m[:, ::8, ::8, ::8]), ("slice_k", m[:, 64])):
+        ref = fx[key]
+        out[key] = {"cells": ("every 8th cell along each axis" if key == "sub"
+                              else "the plane at axis-3 index 64"),
+                    "rel_l2": fmt(np.linalg.norm((got - ref).reshape(8, -1), axis=1) /
+                                  np.linalg.norm(ref.reshape(8, -1), axis=1)),
+                    "max_abs": fmt(np.abs(got - ref).reshape(8, -1).max(axis=1))}
+        if "ulp_" + key in fx:
+            out[key]["oracle_one_ulp_spread"] = fmt(fx["ulp_" + key])
+    return out
 
 
 def measured_traffic(precision, n):
