@@ -51,21 +51,46 @@ constexpr int TX = 16, TY = 16, HALO = 3;
 constexpr int PX = TX + 2 * HALO, PY = TY + 2 * HALO;  // 22 x 22 points
 constexpr int NT = TX * TY;
 constexpr int NPTS = PX * PY;
-// the flux buffer: x fluxes at columns -2..17 of rows 0..15, then y fluxes
-// at columns 0..15 of rows -2..17 (one buffer of 20 x 16 points)
-constexpr int FN = (TX + 4) * TY;
-
 // per point values: q of setnu (rho', thermal energy, v0, v1, v2, b0, b1,
 // b2), rho_t, m0, m1, m2
 constexpr int Q0 = 0, RT = 8, M0 = 9, NV = 12;
 
+// LDS layout.  The flux buffer holds the x fluxes at columns -2..17 of rows
+// 0..15, then the y fluxes at columns 0..15 of rows -2..17.  In FP64, 16
+// lanes read one tile row of 16 doubles (128 bytes: every bank once) and
+// the rows keep their natural widths.  In FP32 a half-wave (32 lanes, served
+// together) reads two tile rows; they are rows r and r+8 (see cell_row),
+// and every row pitch is 2 (mod 4) floats, so the two rows' reads fall
+// 8 * pitch = 16 (mod 32) banks apart and never share a bank.  (Rows r and
+// r+1 at the pitch of 22 shared six banks: two cycles per read.)  The
+// FP32 layout stays under 40 KB, four blocks per CU.
+template <typename C>
+struct Lay {
+  static constexpr bool F32 = sizeof(C) == 4;
+  static constexpr int FX = F32 ? TX + 6 : TX + 4;  // x-flux rows
+  static constexpr int FY = F32 ? TX + 2 : TX;      // y-flux rows
+  static constexpr int FN = (TY * FX > (TY + 4) * FY) ? TY * FX : (TY + 4) * FY;
+  static constexpr int T01 = F32 ? TX + 2 : TX;     // rows of tx01
+};
+
 template <typename C>
 struct Smem {
+  using L = Lay<C>;
   C pv[NV][PY][PX];
-  C F[8][FN];
-  C tx01[2][TY + 2][TX];  // Tm_{0,1}, Tb_{0,1} of rows -1..16
-  C ty10[2][TY][TX + 2];  // Tm_{1,0}, Tb_{1,0} of columns -1..16
+  C F[8][L::FN];
+  C tx01[2][TY + 2][L::T01];  // Tm_{0,1}, Tb_{0,1} of rows -1..16
+  C ty10[2][TY][TX + 2];      // Tm_{1,0}, Tb_{1,0} of columns -1..16
 };
+
+// tile row of thread tid: tid / 16 in FP64; in FP32 lanes 0-15, 16-31,
+// 32-47, 48-63 of wave w take rows 2w, 2w+8, 2w+1, 2w+9
+template <typename C>
+__device__ __forceinline__ int cell_row(int tid) {
+  if constexpr (Lay<C>::F32)
+    return 2 * (tid >> 6) + ((tid >> 5) & 1) + 8 * ((tid >> 4) & 1);
+  else
+    return tid / TX;
+}
 
 __host__ __device__ inline long long nblocks(const GridDesc& g) {
   const long long ntx = (g.Mhi[0] + 1 + TX - 1) / TX;  // tiles from grid column 0
@@ -205,7 +230,8 @@ __device__ __forceinline__ void plane_tile(const AxisArgs& a, Smem<C>& sm, C& nu
   const double* __restrict__ Bg = a.bg;
   const double gamma = a.gamma;
   const int tid = threadIdx.x;
-  const int tx = tid % TX, ty = tid / TX;
+  using LY = Lay<C>;
+  const int tx = tid % TX, ty = cell_row<C>(tid);
 
   // block -> tile: each XCD takes one contiguous range of tiles, so the
   // halos a tile shares with its neighbours are fetched into one L2
@@ -240,7 +266,7 @@ __device__ __forceinline__ void plane_tile(const AxisArgs& a, Smem<C>& sm, C& nu
       C F[NE], v[NDIM];
       flux_pt<NDIR, NDIM, C>(pt, 0, gamma, F, v);
 #pragma unroll
-      for (int n = 0; n < NE; n++) sm.F[n][(iy - HALO) * (TX + 4) + ix - 1] = F[n];
+      for (int n = 0; n < NE; n++) sm.F[n][(iy - HALO) * LY::FX + ix - 1] = F[n];
     }
     if (in_fy) {
       C v[NDIM];
@@ -293,7 +319,7 @@ __device__ __forceinline__ void plane_tile(const AxisArgs& a, Smem<C>& sm, C& nu
     C Tm[3] = {0, 0, 0}, Tb[3] = {0, 0, 0};
     cell_axis<0, C>(
         R, pbg, gamma, cx, g.Mlo[0], g.Mhi[0], xlo, xhi, dxx, ctx, a.visc_on, valid, numax,
-        [&](int n, int d) { return sm.F[n][ty * (TX + 4) + tx + 2 + d]; },
+        [&](int n, int d) { return sm.F[n][ty * LY::FX + tx + 2 + d]; },
         [&](int p, int d) { return sm.pv[Q0 + p][py][px + d]; },
         [&](int d) { return sm.pv[RT][py][px + d]; },
         [&](int l, int d) { return sm.pv[M0 + l][py][px + d]; }, Tm, Tb);
@@ -324,7 +350,7 @@ __device__ __forceinline__ void plane_tile(const AxisArgs& a, Smem<C>& sm, C& nu
     const int ix = i % PX, iy = i / PX;
     if (iy >= 1 && iy <= PY - 2 && ix >= HALO && ix < HALO + TX) {
 #pragma unroll
-      for (int n = 0; n < NE; n++) sm.F[n][(iy - 1) * TX + ix - HALO] = Fy[k][n];
+      for (int n = 0; n < NE; n++) sm.F[n][(iy - 1) * LY::FY + ix - HALO] = Fy[k][n];
     }
   }
   __syncthreads();
@@ -335,7 +361,7 @@ __device__ __forceinline__ void plane_tile(const AxisArgs& a, Smem<C>& sm, C& nu
     C Tm[3] = {0, 0, 0}, Tb[3] = {0, 0, 0};
     cell_axis<1, C>(
         Ry, pbg, gamma, cy, g.Mlo[1], g.Mhi[1], ylo, yhi, dxy, cty, a.visc_on, valid, numax,
-        [&](int n, int d) { return sm.F[n][(ty + 2 + d) * TX + tx]; },
+        [&](int n, int d) { return sm.F[n][(ty + 2 + d) * LY::FY + tx]; },
         [&](int p, int d) { return sm.pv[Q0 + p][py + d][px]; },
         [&](int d) { return sm.pv[RT][py + d][px]; },
         [&](int l, int d) { return sm.pv[M0 + l][py + d][px]; }, Tm, Tb);
