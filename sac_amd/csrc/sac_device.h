@@ -27,13 +27,6 @@
 #pragma once
 #include "sac_kernels.h"
 
-// mesh cells per thread of the line kernels along x and along y/z
-#ifndef SAC_CK_X
-#define SAC_CK_X 2
-#endif
-#ifndef SAC_CK_YZ
-#define SAC_CK_YZ 2
-#endif
 
 // keeps the machine scheduler from hoisting the next phase's loads above
 // the current phase's arithmetic, which bounds the number of live values
@@ -314,9 +307,10 @@ __device__ __forceinline__ long long xcd_block(int xcd) {
 // and the CK+1 face ratios are evaluated once for the CK cells.  The
 // contribution is added to the rate accumulator R (written on K=0).
 
+// mesh cells per thread of a line group: 2 along every axis
 template <int K>
 struct LineGroup {
-  static constexpr int CK = (K == 0) ? SAC_CK_X : SAC_CK_YZ;
+  static constexpr int CK = 2;
 };
 
 // One group of CK consecutive mesh cells along K starting at c[K] (other

@@ -376,7 +376,7 @@ int launch_axis(const AxisArgs& a, int axis, hipStream_t st) {
   for (int d = 0; d < 3; d++) {
     long long e = g.Mhi[d] - g.Mlo[d] + 1;
     if (d == axis) {
-      const int ck = (axis == 0) ? SAC_CK_X : SAC_CK_YZ;
+      const int ck = (axis == 0) ? LineGroup<0>::CK : LineGroup<1>::CK;
       e = (e + ck - 1) / ck;
     }
     n *= e;
