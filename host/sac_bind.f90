@@ -11,6 +11,7 @@ module sac_bind
   integer(c_int), parameter :: SAC_BC_SYMM = 4, SAC_BC_SYMM0 = 5, SAC_BC_ASYMM = 6
   integer(c_int), parameter :: SAC_BC_CONT1 = 7, SAC_BC_FIXED1 = 8, SAC_BC_GRAD1 = 9
   integer(c_int), parameter :: SAC_BC_CONTCD4 = 10
+  integer(c_int), parameter :: SAC_BC_ZERO = 11
   integer(c_int), parameter :: SAC_ADV_ONESTEP = 1, SAC_ADV_TWOSTEP = 2, SAC_ADV_THREESTEP = 3, &
                                 SAC_ADV_FOURSTEP = 4, SAC_ADV_STERCK = 5, SAC_ADV_JAMESON = 6
   integer(c_int), parameter :: SAC_PREC_FP64 = 0

@@ -12,7 +12,7 @@
 ! Configurations outside the GPU path stop with a message, as the
 ! reference's die() does: other schemes or splittings, a resistivity given
 ! by the user's specialeta (eqpar(eta_)<0), smallfix, generalised
-! coordinates, axial symmetry and the boundary types zero and special.
+! coordinates, axial symmetry and the boundary type special.
 !=============================================================================
 module host_state
   use iso_c_binding
@@ -605,6 +605,8 @@ contains
       bc_code = SAC_BC_GRAD1
     case ('contCD4')
       bc_code = SAC_BC_CONTCD4
+    case ('zero')
+      bc_code = SAC_BC_ZERO
     case default
       call die('Error in GetBoundary: boundary type ' // trim(s) // ' is not implemented!')
       bc_code = 0

@@ -41,6 +41,8 @@ extern "C" {
 #define SAC_BC_FIXED1 8  /* extrapolated once, then restored */
 #define SAC_BC_GRAD1 9   /* extrapolated once, then stored difference + edge value */
 #define SAC_BC_CONTCD4 10 /* mirror about the edge cell (src/vacgrid.t:260-290) */
+#define SAC_BC_ZERO 11    /* contCD4's copy in primitive variables, the whole grid
+                             converted before and after (src/vacgrid.t:294-345) */
 
 /* typeadvance codes (src/vac.t:406-480) */
 #define SAC_ADV_ONESTEP 1

@@ -90,6 +90,7 @@ struct sac_oracle {
   double *tmp, *tmp2;
   double *cd_v, *cd_f, *cd_fb;
   double *gf_fb;
+  double *bd_p;     /* pressure of primitive() ('zero' boundaries) */
   double *csound2, *cfast2;
   double *gc_cmax;
   double *av_nushk; /* [3][n] */
@@ -256,6 +257,7 @@ sac_oracle *oracle_new(int ndim, int ndir, const int *nx, const double *eqpar,
   o->cd_f = zalloc(n);
   o->cd_fb = zalloc(n);
   o->gf_fb = zalloc(n);
+  o->bd_p = zalloc(n);
   o->csound2 = zalloc(n);
   o->cfast2 = zalloc(n);
   o->gc_cmax = zalloc(n);
@@ -297,7 +299,7 @@ sac_oracle *oracle_new(int ndim, int ndir, const int *nx, const double *eqpar,
 void oracle_free(sac_oracle *o) {
   if (!o) return;
   double *ps[] = {o->x, o->dx, o->w, o->fixB, o->tmp, o->tmp2, o->cd_v, o->cd_f, o->cd_fb,
-                  o->gf_fb, o->csound2, o->cfast2, o->gc_cmax, o->av_nushk, o->av_rhoL,
+                  o->gf_fb, o->bd_p, o->csound2, o->cfast2, o->gc_cmax, o->av_nushk, o->av_rhoL,
                   o->av_rhoR, o->av_rhoC, o->av_VL, o->av_VR, o->av_BL, o->av_BR, o->av_L,
                   o->av_R, o->av_C, o->av_nuL, o->av_nuR, o->sn_cmax, o->sn_tmp_nu,
                   o->sn_tmp_nuI, o->sn_d1R, o->sn_d1L, o->sn_d3R, o->sn_d3L, o->sn_md3R,
@@ -424,7 +426,10 @@ int oracle_get_dx(const sac_oracle *o, double *dx) {
 
 /* ------------------------------------------------------------------ */
 /* getboundary (src/vacgrid.t:195-512) for cont/fixed/periodic/symm/asymm,
- * cont1/fixed1/grad1 and contCD4 */
+ * cont1/fixed1/grad1, contCD4 and zero */
+
+static void primitive(sac_oracle *o, double *w);
+static void conserve(sac_oracle *o, double *w);
 
 static void getboundary(sac_oracle *o, double *w) {
   int nd = o->ndim;
@@ -491,6 +496,19 @@ static void getboundary(sac_oracle *o, double *w) {
           if (ix == lo) F(w, iw) = FSH(w, iw, idim, src_lo - ix);
           else if (ix == hi) F(w, iw) = FSH(w, iw, idim, src_hi - ix);
         }
+      } else if (tb == OR_BC_ZERO) {
+        /* the contCD4 copy of the two ghost layers, made in primitive
+         * variables: the whole grid is converted to primitive variables
+         * before and back to conservative ones after (src/vacgrid.t:294-345) */
+        int lo = r.lo[idim - 1], hi = r.hi[idim - 1];
+        int src_lo = upper ? lo - 2 : hi + 3, src_hi = upper ? lo - 3 : hi + 2;
+        primitive(o, w);
+        FOR_BOX(r) {
+          int ix = (idim == 1) ? i1 : (idim == 2) ? i2 : i3;
+          if (ix == lo) F(w, iw) = FSH(w, iw, idim, src_lo - ix);
+          else if (ix == hi) F(w, iw) = FSH(w, iw, idim, src_hi - ix);
+        }
+        conserve(o, w);
       } else if (tb == OR_BC_SYMM || tb == OR_BC_SYMM0 || tb == OR_BC_ASYMM) {
         int ixe = upper ? 2 * r.lo[idim - 1] - 1 : 2 * r.hi[idim - 1] + 1;
         FOR_BOX(r) {
@@ -565,6 +583,34 @@ static void getpthermal(sac_oracle *o, const double *w, box ix, double *p) {
     C(p) = C(p) + 0.5 * sb + sbb;
     C(p) = (g - 1.0) * (F(w, o->e_) - C(p));
   }
+}
+
+/* primitive (src/vacphys0.t.mhd:31-53) over the whole grid: e <- the
+ * thermal pressure, m <- m/rho_t */
+static void primitive(sac_oracle *o, double *w) {
+  box g = grid_box(o);
+  double *p = o->bd_p;
+  getpthermal(o, w, g, p);
+  FOR_BOX(g) { F(w, o->e_) = C(p); }
+  for (int c = 1; c <= o->ndir; c++) FOR_BOX(g) { F(w, o->m0_ + c) = F(w, o->m0_ + c) / RHOT(w); }
+}
+
+/* conserve (src/vacphys0.t.mhd:5-28) over the whole grid:
+ * e <- p/(gamma-1) + half*(rho_t*sum v^2 + sum b^2) + sum b*bg, m <- rho_t*v */
+static void conserve(sac_oracle *o, double *w) {
+  box g = grid_box(o);
+  double gm1 = eq_gamma(o) - 1.0;
+  FOR_BOX(g) {
+    double sv = 0.0, sb = 0.0, sbb = 0.0;
+    for (int c = 1; c <= o->ndir; c++) {
+      double v = F(w, o->m0_ + c), b = F(w, o->b0_ + c);
+      sv = (c == 1) ? v * v : sv + v * v;
+      sb = (c == 1) ? b * b : sb + b * b;
+      sbb = (c == 1) ? b * F(w, o->bg0_ + c) : sbb + b * F(w, o->bg0_ + c);
+    }
+    F(w, o->e_) = F(w, o->e_) / gm1 + 0.5 * (RHOT(w) * sv + sb) + sbb;
+  }
+  for (int c = 1; c <= o->ndir; c++) FOR_BOX(g) { F(w, o->m0_ + c) = RHOT(w) * F(w, o->m0_ + c); }
 }
 
 /* getcsound2 (src/vacphys0.t.mhd:137-160) */

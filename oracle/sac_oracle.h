@@ -35,7 +35,8 @@ enum {
   OR_BC_CONT1 = 7,
   OR_BC_FIXED1 = 8,
   OR_BC_GRAD1 = 9,
-  OR_BC_CONTCD4 = 10
+  OR_BC_CONTCD4 = 10,
+  OR_BC_ZERO = 11
 };
 
 /* typeadvance codes (src/vac.t:406-480) */

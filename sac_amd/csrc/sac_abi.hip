@@ -94,6 +94,8 @@ struct Dev {
   // their values are stored (evolving fields, background)
   bool cont1_e = false, ext1_e = false, cont1_b = false, ext1_b = false;
   bool initialized = false;  // getboundary's 'initialized' (src/vacgrid.t:209)
+  bool any_zero = false;     // some 'zero' boundary: region-by-region fills
+  std::vector<int> bce_h, bcb_h;  // boundary codes [var][6] (evolving, background)
   // an upload after the first getboundary replaced the background's ghost
   // cells with the caller's: the next getboundary fills them again
   bool bg_ghosts_stale = false;
@@ -195,6 +197,37 @@ void ghost_fill_evolving(Dev& d, int slot) {
   t_begin(d, SAC_K_GHOST, &e0);
   sac::launch_ghost(a, d.stream);
   t_end(d, SAC_K_GHOST, e0);
+}
+
+// getboundary region by region, variable by variable, as the reference
+// does (src/vacgrid.t:195-512): the path of grids with 'zero' boundaries,
+// whose fills convert the whole register to primitive variables and back.
+// The background's own fills are made when its ghosts are not yet filled;
+// afterwards only its 'zero' fills recur (for their conversions of the
+// register).  The conversions move mesh cells by rounding, so the register's
+// reductions are taken again before their next use.
+void ghost_fill_sequential(Dev& d, int slot, bool fill_bg) {
+  for (int iB = 0; iB < 2 * d.ndim; iB++) {
+    for (int iw = 0; iw < d.ne + d.nbg; iw++) {
+      const bool ev = iw < d.ne;
+      const int code = ev ? d.bce_h[(size_t)iw * 6 + iB] : d.bcb_h[(size_t)(iw - d.ne) * 6 + iB];
+      if (!ev && !fill_bg && code != SAC_BC_ZERO) continue;
+      sac::RegionArgs r{};
+      r.g = d.g;
+      r.prec = ev ? d.prec : sac::PREC_FP64;
+      r.f = ev ? (void*)((char*)d.reg[slot] + d.esz * (size_t)iw * d.g.fs)
+               : (void*)(d.bg + (size_t)(iw - d.ne) * d.g.fs);
+      r.code = code;
+      r.D = iB / 2;
+      r.side = iB % 2;
+      r.keep_fixed = (!d.initialized && d.fullgridini) ? 1 : 0;
+      const bool zero = code == SAC_BC_ZERO;
+      if (zero) sac::launch_prim_cons(d.g, d.prec, d.reg[slot], d.bg, d.gamma, 1, d.stream);
+      if (!(zero && !ev && !fill_bg)) sac::launch_region_fill(r, d.stream);
+      if (zero) sac::launch_prim_cons(d.g, d.prec, d.reg[slot], d.bg, d.gamma, 0, d.stream);
+    }
+  }
+  d.red_valid[slot] = false;
 }
 
 void cmax_reduce(Dev& d, int slot) {
@@ -310,7 +343,10 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   t_end(d, SAC_K_FINAL, e0);
   d.red_valid[out] = true;
   if (acc_out >= 0) d.red_valid[acc_out] = false;
-  ghost_fill_evolving(d, out);
+  if (d.any_zero)
+    ghost_fill_sequential(d, out, false);
+  else
+    ghost_fill_evolving(d, out);
 }
 
 int check_handle(sac_handle h) {
@@ -431,9 +467,21 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
   for (int iw = 0; iw < nw; iw++)
     for (int iB = 0; iB < 2 * ndim; iB++) {
       int c = typeB[iw * 2 * ndim + iB];
-      if (c < SAC_BC_CONT || c > SAC_BC_CONTCD4)
+      if (c < SAC_BC_CONT || c > SAC_BC_ZERO)
         return fail(SAC_E_UNSUPPORTED, "boundary code %d (var %d, region %d)", c, iw + 1, iB + 1);
     }
+  {
+    // 'zero' fills run region by region (ghost_fill_sequential), which
+    // restores no stored values: not with fixed, fixed1 or grad1
+    bool zero = false, stored = false;
+    for (int i = 0; i < nw * 2 * ndim; i++) {
+      zero |= typeB[i] == SAC_BC_ZERO;
+      stored |= typeB[i] == SAC_BC_FIXED || typeB[i] == SAC_BC_FIXED1 || typeB[i] == SAC_BC_GRAD1;
+    }
+    if (zero && stored)
+      return fail(SAC_E_UNSUPPORTED,
+                  "'zero' boundaries together with fixed, fixed1 or grad1 are not built");
+  }
   // periodic pairing must be consistent (src/vacgrid.t:109-120)
   for (int d = 0; d < ndim; d++) {
     int lo = typeB[0 * 2 * ndim + 2 * d], hi = typeB[0 * 2 * ndim + 2 * d + 1];
@@ -541,11 +589,15 @@ int sac_dev_init(int ndim, int ndir, const int* nx, int nw, const int* iws, int 
     }
   }
   // boundary codes per register
-  std::vector<int> bce((size_t)ne * 6, SAC_BC_CONT), bcb((size_t)d->nbg * 6, SAC_BC_CONT);
+  std::vector<int>& bce = d->bce_h;
+  std::vector<int>& bcb = d->bcb_h;
+  bce.assign((size_t)ne * 6, SAC_BC_CONT);
+  bcb.assign((size_t)d->nbg * 6, SAC_BC_CONT);
   for (int iw = 0; iw < nw; iw++)
     for (int iB = 0; iB < 2 * ndim; iB++) {
       int c = typeB[iw * 2 * ndim + iB];
       const bool c1 = c == SAC_BC_CONT1, x1 = c == SAC_BC_FIXED1 || c == SAC_BC_GRAD1;
+      d->any_zero |= c == SAC_BC_ZERO;
       if (iw < ne) {
         bce[(size_t)iw * 6 + iB] = c;
         d->cont1_e |= c1;
@@ -729,6 +781,15 @@ int sac_dev_save_end(sac_handle h, double* w) {
 int sac_dev_getboundary(sac_handle h) {
   if (int rc = check_handle(h)) return rc;
   Dev& d = *(Dev*)h;
+  if (d.any_zero) {
+    ghost_fill_sequential(d, SAC_SLOT_W, !d.initialized || d.bg_ghosts_stale);
+    d.initialized = true;  // no stored values in this path
+    d.bg_ghosts_stale = false;
+    d.bg_host.clear();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(d.stream));
+    return 0;
+  }
   if (!d.initialized || d.bg_ghosts_stale) {
     // background ghosts: on the first call (the background is static, so
     // the reference fills them once) and after an upload replaced them

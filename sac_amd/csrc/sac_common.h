@@ -32,7 +32,8 @@ enum BcCode : int {
   BC_CONT1 = 7,
   BC_FIXED1 = 8,
   BC_GRAD1 = 9,
-  BC_CONTCD4 = 10
+  BC_CONTCD4 = 10,
+  BC_ZERO = 11
 };
 
 struct GridDesc {

@@ -69,6 +69,25 @@ struct GhostArgs {
   int general;         // some fill has two sources (cont1; fixed1/grad1 before storage)
 };
 
+// One region's fill of one field, in the reference's order (iB, then the
+// variable), for grids with 'zero' boundaries (launch_region_fill): the
+// ghost layers beyond edge `side` of axis D across the whole grid along the
+// other axes, from the field's current values.  'zero' takes contCD4's copy
+// (the caller converts the register to primitive variables around it).
+struct RegionArgs {
+  GridDesc g;
+  int prec;            // storage type of f
+  void* f;             // the field
+  int code;            // boundary code
+  int D, side;         // axis and edge (0 lower, 1 upper)
+  int keep_fixed;      // first fill with fullgridini: keep the file's values
+};
+void launch_region_fill(const RegionArgs& a, hipStream_t st);
+// primitive (to_primitive = 1) or conserve (0) over the whole grid of a
+// register (src/vacphys0.t.mhd:5-53)
+void launch_prim_cons(const GridDesc& g, int prec, void* w, const double* bg, double gamma,
+                      int to_primitive, hipStream_t st);
+
 // resistive sources with a constant eta > 0 (sac_resist.hip)
 struct ResistArgs {
   GridDesc g;

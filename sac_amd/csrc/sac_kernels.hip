@@ -257,6 +257,115 @@ __global__ void __launch_bounds__(256) k_ghost_store(GhostArgs a, S* __restrict_
 }
 
 // ---------------------------------------------------------------------
+// kernel 4b: grids with 'zero' boundaries fill their ghost regions one
+// after another, as the reference does (src/vacgrid.t:195-512), because
+// every 'zero' fill converts the whole grid to primitive variables and back
+// (src/vacgrid.t:294-345), which moves every cell by rounding: no ghost value
+// can be traced back to mesh cells as k_ghost does.
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_region_fill(RegionArgs a) {
+  const GridDesc& g = a.g;
+  const int D = a.D;
+  const int nlay = (a.side ? g.G[D] - 1 - g.Mhi[D] : g.Mlo[D]);  // ghost layers
+  int ext[3] = {g.G[0], g.G[1], g.G[2]};
+  ext[D] = nlay;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ext[0] * ext[1] * ext[2]) return;
+  const int lo3[3] = {0, 0, 0};
+  int c[3];
+  decode_box(t, lo3, ext, c);
+  const int lo = a.side ? g.Mhi[D] + 1 : 0, hi = lo + nlay - 1;  // the region along D
+  c[D] += lo;
+  S* f = (S*)a.f;
+  auto at = [&](int cd) {
+    int e[3] = {c[0], c[1], c[2]};
+    e[D] = cd;
+    return f[lin(g, e[0], e[1], e[2])];
+  };
+  const int N = g.Mhi[D] - g.Mlo[D] + 1;
+  const int ixe = a.side ? g.Mhi[D] : g.Mlo[D];  // edge cell
+  const long long i = lin(g, c[0], c[1], c[2]);
+  switch (a.code) {
+    case BC_PERIODIC:
+      f[i] = at(c[D] + (a.side ? -N : N));
+      break;
+    case BC_SYMM:
+    case BC_SYMM0:
+      f[i] = at(a.side ? (2 * g.Mhi[D] + 1 - c[D]) : (2 * g.Mlo[D] - 1 - c[D]));
+      break;
+    case BC_ASYMM:
+      f[i] = -at(a.side ? (2 * g.Mhi[D] + 1 - c[D]) : (2 * g.Mlo[D] - 1 - c[D]));
+      break;
+    case BC_CONTCD4:
+    case BC_ZERO: {  // upper ghosts lo <- lo-2, hi <- lo-3; lower lo <- hi+3, hi <- hi+2
+      const int src = (c[D] == lo) ? (a.side ? lo - 2 : hi + 3) : (a.side ? lo - 3 : hi + 2);
+      if (c[D] == lo || c[D] == hi) f[i] = at(src);
+      break;
+    }
+    case BC_FIXED:
+      if (!a.keep_fixed) f[i] = at(ixe);  // the first fill is cont's
+      break;
+    case BC_FIXED1:
+    case BC_GRAD1:
+      if (a.keep_fixed) break;
+      [[fallthrough]];
+    case BC_CONT1: {
+#pragma clang fp contract(off)
+      const int k = abs(c[D] - ixe);
+      const int ixf = a.side ? ixe - 1 : ixe + 1;
+      f[i] = (S)((double)(k + 1) * (double)at(ixe) - (double)k * (double)at(ixf));
+      break;
+    }
+    default:  // cont
+      f[i] = at(ixe);
+  }
+}
+
+// primitive: e <- p_thermal (getpthermal, src/vacphys0.t.mhd:163-183),
+// m <- m/rho_t; conserve: e <- p/(gamma-1) + half*(rho_t*sum v^2 + sum b^2)
+// + sum b*bg, m <- rho_t*v.  Every cell of the grid, ghosts included.
+template <int NDIR, typename S>
+__global__ void __launch_bounds__(256) k_prim_cons(GridDesc g, S* w, const double* bg,
+                                                   double gamma, int to_primitive) {
+#pragma clang fp contract(off)
+  const long long ncell = (long long)g.G[0] * g.G[1] * g.G[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  int c[3];
+  const int lo3[3] = {0, 0, 0};
+  decode_box(t, lo3, g.G, c);
+  const long long i = lin(g, c[0], c[1], c[2]), fs = g.fs;
+  const double rt = (double)w[i] + bg[fs + i];
+  double m[NDIR], sb = 0.0, sbb = 0.0;
+#pragma unroll
+  for (int cd = 0; cd < NDIR; cd++) {
+    m[cd] = (double)w[(1 + cd) * fs + i];
+    const double b = (double)w[(2 + NDIR + cd) * fs + i];
+    sb = (cd == 0) ? b * b : sb + b * b;
+    sbb = (cd == 0) ? b * bg[(2 + cd) * fs + i] : sbb + b * bg[(2 + cd) * fs + i];
+  }
+  const double e = (double)w[(1 + NDIR) * fs + i];
+  if (to_primitive) {
+    double sm = 0.0;
+#pragma unroll
+    for (int cd = 0; cd < NDIR; cd++) sm = (cd == 0) ? m[cd] * m[cd] : sm + m[cd] * m[cd];
+    double p = 0.5 * sm / rt;
+    p = p + 0.5 * sb + sbb;
+    w[(1 + NDIR) * fs + i] = (S)((gamma - 1.0) * (e - p));
+#pragma unroll
+    for (int cd = 0; cd < NDIR; cd++) w[(1 + cd) * fs + i] = (S)(m[cd] / rt);
+  } else {
+    double sv = 0.0;  // the momentum slots hold v
+#pragma unroll
+    for (int cd = 0; cd < NDIR; cd++) sv = (cd == 0) ? m[cd] * m[cd] : sv + m[cd] * m[cd];
+    w[(1 + NDIR) * fs + i] = (S)(e / (gamma - 1.0) + 0.5 * (rt * sv + sb) + sbb);
+#pragma unroll
+    for (int cd = 0; cd < NDIR; cd++) w[(1 + cd) * fs + i] = (S)(rt * m[cd]);
+  }
+}
+
+// ---------------------------------------------------------------------
 // small helpers
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
@@ -422,6 +531,40 @@ void launch_ghost(const GhostArgs& a, hipStream_t st) {
       hipLaunchKernelGGL((k_ghost<double, true>), grid, dim3(bs), 0, st, a);
     else
       hipLaunchKernelGGL((k_ghost<double, false>), grid, dim3(bs), 0, st, a);
+  }
+}
+
+void launch_region_fill(const RegionArgs& a, hipStream_t st) {
+  const GridDesc& g = a.g;
+  const int nlay = a.side ? g.G[a.D] - 1 - g.Mhi[a.D] : g.Mlo[a.D];
+  long long n = (long long)g.G[0] * g.G[1] * g.G[2] / g.G[a.D] * nlay;
+  if (n <= 0) return;
+  const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+  if (a.prec == PREC_MIXED)
+    hipLaunchKernelGGL(k_region_fill<float>, grid, blk, 0, st, a);
+  else
+    hipLaunchKernelGGL(k_region_fill<double>, grid, blk, 0, st, a);
+}
+
+void launch_prim_cons(const GridDesc& g, int prec, void* w, const double* bg, double gamma,
+                      int to_primitive, hipStream_t st) {
+  const long long n = (long long)g.G[0] * g.G[1] * g.G[2];
+  const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+  if (g.ndir == 1) {
+    if (prec == PREC_MIXED)
+      hipLaunchKernelGGL((k_prim_cons<1, float>), grid, blk, 0, st, g, (float*)w, bg, gamma, to_primitive);
+    else
+      hipLaunchKernelGGL((k_prim_cons<1, double>), grid, blk, 0, st, g, (double*)w, bg, gamma, to_primitive);
+  } else if (g.ndir == 2) {
+    if (prec == PREC_MIXED)
+      hipLaunchKernelGGL((k_prim_cons<2, float>), grid, blk, 0, st, g, (float*)w, bg, gamma, to_primitive);
+    else
+      hipLaunchKernelGGL((k_prim_cons<2, double>), grid, blk, 0, st, g, (double*)w, bg, gamma, to_primitive);
+  } else {
+    if (prec == PREC_MIXED)
+      hipLaunchKernelGGL((k_prim_cons<3, float>), grid, blk, 0, st, g, (float*)w, bg, gamma, to_primitive);
+    else
+      hipLaunchKernelGGL((k_prim_cons<3, double>), grid, blk, 0, st, g, (double*)w, bg, gamma, to_primitive);
   }
 }
 

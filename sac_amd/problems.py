@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 BC = {"cont": 1, "fixed": 2, "periodic": 3, "symm": 4, "symm0": 5, "asymm": 6,
-      "cont1": 7, "fixed1": 8, "grad1": 9, "contCD4": 10}
+      "cont1": 7, "fixed1": 8, "grad1": 9, "contCD4": 10, "zero": 11}
 BC_NAMES = {v: k for k, v in BC.items()}
 
 GRAV0 = 3  # grav0_ (1-based)

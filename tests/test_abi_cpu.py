@@ -58,6 +58,17 @@ def test_init_rejects_unsupported_configurations():
     rc = L.sac_dev_init(1, 2, device._ip(nx), 11, device._ip(iws), 6, device._dp(eq), 6, 3, 6,
                         device._dp(dx), device._ip(tb), 0, 4, 0, ctypes.byref(h))
     assert rc == 1 and b"nw" in L.sac_dev_last_error()
+    # 'zero' fills run region by region without stored values: not together
+    # with fixed, fixed1 or grad1; a code past 'zero' is not a boundary type
+    tb[:] = 11
+    tb[3] = 2
+    rc = L.sac_dev_init(1, 2, device._ip(nx), 10, device._ip(iws), 6, device._dp(eq), 6, 3, 6,
+                        device._dp(dx), device._ip(tb), 0, 4, 0, ctypes.byref(h))
+    assert rc == 2 and b"'zero'" in L.sac_dev_last_error()
+    tb[3] = 12
+    rc = L.sac_dev_init(1, 2, device._ip(nx), 10, device._ip(iws), 6, device._dp(eq), 6, 3, 6,
+                        device._dp(dx), device._ip(tb), 0, 4, 0, ctypes.byref(h))
+    assert rc == 2 and b"boundary code 12" in L.sac_dev_last_error()
 
 
 def test_batch_init_rejects_bad_ensembles():

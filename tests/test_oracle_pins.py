@@ -927,3 +927,78 @@ def test_resistive_dt_limit(oracle_mod):
     assert o.getdt_res(1e30) == pytest.approx(0.6 * dxmin ** 2 / 2e-3, rel=1e-14)
     o.getdt_special(1e30)  # getdt_visc sets the common dtdiffpar to 0.25
     assert o.getdt_res(1e30) == pytest.approx(0.25 * dxmin ** 2 / 2e-3, rel=1e-14)
+
+
+def _zero_state(ndim, nx, gamma, seed, moving):
+    """random state for the 'zero' boundary pins: rho' and e of either sign
+    around positive totals, the background positive in every cell (ghosts
+    included, so that the whole-grid conversions never divide by zero);
+    moving=False: no momentum and no perturbed field"""
+    prob, v = _blank(ndim, 3, nx, gamma=gamma, bc="cont")
+    rng = np.random.default_rng(seed)
+    w = prob.w
+    ms = prob.mesh_slices()
+    w[v["rhob"] - 1] = rng.uniform(1.0, 2.0, w[0].shape)
+    w[v["eb"] - 1] = rng.uniform(2.0, 3.0, w[0].shape)
+    for c in range(3):
+        w[v["bg0"] + c] = rng.uniform(-0.5, 0.5, w[0].shape)
+    w[0][ms] = rng.uniform(-0.2, 0.2, w[0][ms].shape)
+    w[v["e"] - 1][ms] = rng.uniform(1.0, 2.0, w[0][ms].shape)
+    if moving:
+        for c in range(3):
+            w[v["m0"] + c][ms] = rng.uniform(-0.3, 0.3, w[0][ms].shape)
+            w[v["b0"] + c][ms] = rng.uniform(-0.3, 0.3, w[0][ms].shape)
+    return prob, v
+
+
+def test_zero_boundary_is_contcd4_when_the_conversions_are_exact(oracle_mod):
+    """'zero' (src/vacgrid.t:294-345) is contCD4's copy made in primitive
+    variables with the whole grid converted before and after.  With gamma=2,
+    no momentum and no perturbed field the conversions are exact (p = e,
+    v = 0), so the fill must equal contCD4's everywhere, bit for bit."""
+    for ndim, nx in ((1, (12,)), (2, (10, 9))):
+        prob, v = _zero_state(ndim, nx, 2.0, seed=ndim, moving=False)
+        prob.typeB[:, :] = BC["zero"]
+        wz = make_oracle(prob, oracle_mod).get_w()
+        prob.typeB[:, :] = BC["contCD4"]
+        wc = make_oracle(prob, oracle_mod).get_w()
+        np.testing.assert_array_equal(wz, wc)
+
+
+def test_zero_boundary_copies_primitive_variables(oracle_mod):
+    """On a general state the 'zero' ghosts hold the contCD4 copy of the
+    primitive variables: rho' and b exactly, v = m/rho_t and the thermal
+    pressure p = (gamma-1)(e - m^2/(2 rho_t) - b^2/2 - b.bg) to rounding; the
+    mesh keeps rho' and b exactly and m, e to rounding (each conversion
+    round trip moves them by a few ulps)."""
+    gamma = 5.0 / 3.0
+    prob, v = _zero_state(1, (12,), gamma, seed=7, moving=True)
+    prob.typeB[:, 0] = BC["cont"]
+    prob.typeB[:, 1] = BC["zero"]  # upper boundary
+    w0 = prob.w.copy()
+    w = make_oracle(prob, oracle_mod).get_w()
+    ne = 8
+    rt = w[0] + w[v["rhob"] - 1]
+    vel = np.stack([w[v["m0"] + c] / rt for c in range(3)])
+    b = np.stack([w[v["b0"] + c] for c in range(3)])
+    bg = np.stack([w[v["bg0"] + c] for c in range(3)])
+    p = (gamma - 1.0) * (w[v["e"] - 1] - 0.5 * rt * (vel ** 2).sum(0) - 0.5 * (b ** 2).sum(0)
+                         - (b * bg).sum(0))
+    sl = lambda i: (0, 0, i)  # noqa: E731
+    mhi = 2 + 12 - 1
+    for ghost, src in ((mhi + 1, mhi - 1), (mhi + 2, mhi - 2)):
+        assert w[(0,) + sl(ghost)] == w[(0,) + sl(src)]
+        for c in range(3):
+            assert w[(v["b0"] + c,) + sl(ghost)] == w[(v["b0"] + c,) + sl(src)]
+            assert vel[(c,) + sl(ghost)] == pytest.approx(vel[(c,) + sl(src)], rel=1e-14, abs=1e-16)
+        assert p[sl(ghost)] == pytest.approx(p[sl(src)], rel=1e-13)
+    m = (slice(None), 0, 0, slice(2, 2 + 12))
+    np.testing.assert_array_equal(w[0][m[1:]], w0[0][m[1:]])
+    for c in range(3):
+        np.testing.assert_array_equal(w[v["b0"] + c][m[1:]], w0[v["b0"] + c][m[1:]])
+        np.testing.assert_allclose(w[v["m0"] + c][m[1:]], w0[v["m0"] + c][m[1:]], rtol=1e-15,
+                                   atol=1e-16)
+    np.testing.assert_allclose(w[v["e"] - 1][m[1:]], w0[v["e"] - 1][m[1:]], rtol=1e-14)
+    # the lower (cont) ghosts copy the edge cell of the converted state
+    for c in range(ne):
+        assert w[(c,) + sl(0)] == w[(c,) + sl(2)]
