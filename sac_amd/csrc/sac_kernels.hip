@@ -136,9 +136,18 @@ static void launch_line(const AxisArgs& A, hipStream_t st) {
                        dim3(line::Tile<K, C>::T), 0, st, A, seg);
 }
 
-// kernel 2: the combining update of every mesh cell (see final_cell)
+// kernel 2: the combining update of every mesh cell (see final_cells), FV
+// consecutive cells along x per thread
+template <int NDIM, int NDIR, typename S, typename C>
+struct FinalV {
+  // FP32 storage: two cells per thread, so that neighbouring cells' loads
+  // pair into 8-byte ones (FP64 values already are)
+  static constexpr int V = (sizeof(S) == 4) ? 2 : 1;
+};
+
 template <int NDIM, int NDIR, typename S, typename C>
 __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
+  constexpr int V = FinalV<NDIM, NDIR, S, C>::V;
   const GridDesc& g = a.g;
   const long long t = xcd_block(a.xcd) * blockDim.x + threadIdx.x;
   int lo[3], ext[3];
@@ -147,6 +156,7 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
     lo[d] = g.Mlo[d];
     ext[d] = g.Mhi[d] - g.Mlo[d] + 1;
   }
+  ext[0] = (ext[0] + V - 1) / V;
   const bool active = t < (long long)ext[0] * ext[1] * ext[2];
   double redv[2 * NDIM];
 #pragma unroll
@@ -154,7 +164,9 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
   if (active) {
     int c[3];
     decode_box(t, lo, ext, c);
-    final_cell<NDIM, NDIR, S, C>(a, g, c, redv);
+    c[0] = lo[0] + V * (c[0] - lo[0]);
+    const int nv = (V == 1) ? 1 : min(V, g.Mhi[0] - c[0] + 1);
+    final_cells<NDIM, NDIR, S, C, V>(a, g, c, nv, redv);
   }
   block_max_atomic<2 * NDIM>(redv, a.red_out);
 }
@@ -507,7 +519,12 @@ void launch_plane(const AxisArgs& a, hipStream_t st) {
 }
 
 void launch_final(const FinalArgs& a, hipStream_t st) {
-  dispatch(a.prec, a.g.ndim, a.g.ndir, 1, 0, &a, st, mesh_cells(a.g));
+  // threads: mesh cells, two per thread along x in FP32 storage
+  const GridDesc& g = a.g;
+  const int V = (a.prec == PREC_MIXED) ? 2 : 1;
+  const long long n = (long long)((g.Mhi[0] - g.Mlo[0] + 1 + V - 1) / V) *
+                      (g.Mhi[1] - g.Mlo[1] + 1) * (g.Mhi[2] - g.Mlo[2] + 1);
+  dispatch(a.prec, g.ndim, g.ndir, 1, 0, &a, st, n);
 }
 
 void launch_cmax(const CmaxArgs& a, hipStream_t st) {

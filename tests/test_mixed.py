@@ -44,18 +44,21 @@ def _stage(dev_mod, oracle_mod, prob):
             per_var_errors(prob, got, ref, ne))
 
 
-def test_mixed_stage_stratified(dev_mod, oracle_mod):
-    """one stage from identical inputs: increments at FP32 resolution"""
+@pytest.mark.parametrize("nxyz", [(12, 10, 8), (13, 10, 8)])
+def test_mixed_stage_stratified(dev_mod, oracle_mod, nxyz):
+    """one stage from identical inputs: increments at FP32 resolution (an odd
+    x extent leaves the combining update's last cell pair with one cell)"""
     inc, st = _stage(dev_mod, oracle_mod,
-                     problems.stratified(n=12, nxyz=(12, 10, 8), amp=3e-2))
+                     problems.stratified(n=nxyz[0], nxyz=nxyz, amp=3e-2))
     assert max(inc) <= 1e-6, inc
     assert max(st) <= 1e-6, st
 
 
-def test_mixed_stage_shock_tube(dev_mod, oracle_mod):
+@pytest.mark.parametrize("n", [64, 63])
+def test_mixed_stage_shock_tube(dev_mod, oracle_mod, n):
     """one stage across discontinuities: states at FP32 resolution; the
     increments of the smallest fields (e, b2 near the jump) to 1e-4"""
-    inc, st = _stage(dev_mod, oracle_mod, problems.brio_wu(n=64))
+    inc, st = _stage(dev_mod, oracle_mod, problems.brio_wu(n=n))
     assert max(st) <= 1e-5, st
     assert max(inc) <= 1e-4, inc
 
@@ -73,6 +76,24 @@ def test_mixed_stratified_free_running(dev_mod, oracle_mod):
     errs = per_var_errors(prob, w, ref, 8)
     assert max(errs) <= 1e-5, errs
     np.testing.assert_allclose(dts, rdts, rtol=1e-3)
+
+
+def test_mixed_odd_extent_free_running(dev_mod, oracle_mod):
+    """odd x extent: the cmax reductions of the combining update count the
+    last cell of every row once and nothing past the mesh, so the
+    free-running dt follows the oracle's"""
+    prob = problems.stratified(n=15, nxyz=(15, 12, 9), amp=3e-2)
+    steps = 30
+    ref, rdts, _ = oracle_run(prob, oracle_mod, steps)
+    dev = dev_mod.Device(prob, precision="mixed")
+    dev.upload(prob.w)
+    dev.getboundary()
+    dts = [dev.step(prob.courantpar) for _ in range(steps)]
+    w = dev.download()
+    dev.close()
+    errs = per_var_errors(prob, w, ref, 8)
+    assert max(errs) <= 1e-5, errs
+    np.testing.assert_allclose(dts, rdts, rtol=1e-4)
 
 
 def test_mixed_brio_wu_prescribed_dt(dev_mod, oracle_mod):
