@@ -19,8 +19,24 @@ the same configuration without the oracle:
              contraction and advanced with the same dt sequence, on the
              full state: the reproducibility floor of the algorithm here
 
-Takes about 50 minutes on one core (two 200-step runs at 128^3).  Run from
+Takes about 2.5 hours on one core (two 200-step runs at 128^3, ~20 s a step).  Run from
 the repository root:  python tests/golden/make_stratified128.py
+
+`--ulp` adds to an existing fixture the spread of the oracle itself under a
+one-ulp change of its initial state (every evolving mesh value moved by -1,
+0 or +1 ulp), for five seeds, in runs of about 75 minutes each, side by
+side:
+
+  ulp_sub, ulp_slice_k, ulp_probes
+             per-variable rel-L2 between a perturbed run (the same dt
+             sequence) and `sub`, `slice_k`, `probes`, the maximum over the
+             seeds (ulp_*_all: each seed; ulp_seeds: the seeds): how far
+             apart two FP64 runs of this configuration end when their
+             inputs differ by rounding alone.  The response to rounding is
+             dominated by a few switches of setnu's maxima and differs
+             several-fold between seeds.
+  ulp_dts    the dt sequence of the first seed's run when it chooses its own
+             time steps
 """
 import os
 import sys
@@ -33,7 +49,7 @@ sys.path.insert(0, ROOT)
 
 from oracle import oracle as om  # noqa: E402
 from sac_amd import problems  # noqa: E402
-from tests.helpers import make_oracle, mesh  # noqa: E402
+from tests.helpers import make_oracle, mesh, ulp_perturbed  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NAME = "stratified_128_200steps"
@@ -84,5 +100,72 @@ def main():
     print(NAME, "bytes", os.path.getsize(path), f"{time.time() - t0:.0f} s")
 
 
+def _ulp_leg(args):
+    """one oracle run from a one-ulp start (seed): its summary under the
+    fixture's dt sequence (free=False), or its own dt sequence (free=True)"""
+    seed, free, dts = args
+    om.build()
+    prob = problems.stratified(n=N)
+    prob.w = ulp_perturbed(prob, seed)
+    o = make_oracle(prob, om)
+    out = []
+    for i in range(STEPS):
+        if free:
+            out.append(o.step(prob.courantpar))
+        else:
+            o.advance(float(dts[i]))
+    return (summarize(prob, o.get_w()) if not free else None), np.array(out)
+
+
+KEYS = ("sub", "slice_k", "probes")
+
+
+def add_ulp_spread(seeds, free_seed=20261203):
+    """Add the one-ulp starts of `seeds` (fixed dt) that the fixture does not
+    hold yet, and the free-running dt sequence of free_seed if missing; each
+    run in a process of its own.  ulp_<key> is the per-variable maximum over
+    the seeds held, ulp_<key>_all the value of each seed."""
+    from multiprocessing import Pool
+    path = os.path.join(HERE, NAME + ".npz")
+    fx = dict(np.load(path))
+    have = [int(x) for x in fx.get("ulp_seeds", [])]
+    if "ulp_seeds" not in fx and "ulp_sub" in fx:  # one seed, stored before seeds were listed
+        have = [free_seed]
+        for key in KEYS:
+            fx["ulp_" + key + "_all"] = fx["ulp_" + key][None, :]
+    new = [sd for sd in seeds if sd not in have]
+    jobs = [(sd, False, fx["dts"]) for sd in new]
+    if "ulp_dts" not in fx:
+        jobs.append((free_seed, True, None))
+    t0 = time.time()
+    with Pool(max(1, len(jobs))) as pool:
+        res = pool.map(_ulp_leg, jobs)
+
+    def rel(a, b):
+        return np.array([np.linalg.norm(a[i] - b[i]) / max(np.linalg.norm(b[i]), 1e-300)
+                         for i in range(a.shape[0])])
+    for (sd, free, _), (s, dts_free) in zip(jobs, res):
+        if free:
+            fx["ulp_dts"] = dts_free
+            continue
+        for key in KEYS:
+            v = rel(s[key], fx[key])[None, :]
+            allk = "ulp_" + key + "_all"
+            fx[allk] = np.concatenate([fx[allk], v]) if allk in fx else v
+        have.append(sd)
+    fx["ulp_seeds"] = np.array(have, dtype=np.int64)
+    for key in KEYS:
+        fx["ulp_" + key] = fx["ulp_" + key + "_all"].max(axis=0)
+        print(f"  one-ulp spread, {key}, max over {len(have)} starts: " +
+              " ".join(f"{x:.1e}" for x in fx["ulp_" + key]))
+    r = np.abs(fx["ulp_dts"] - fx["dts"]) / fx["dts"]
+    print(f"  one-ulp spread of the free-running dt: max {r.max():.1e} at step {r.argmax()}")
+    np.savez_compressed(path, **fx)
+    print(NAME, "bytes", os.path.getsize(path), f"{time.time() - t0:.0f} s")
+
+
 if __name__ == "__main__":
-    main()
+    if "--ulp" in sys.argv[1:]:
+        add_ulp_spread([20261203, 1, 2, 3, 4])
+    else:
+        main()

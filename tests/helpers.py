@@ -37,3 +37,15 @@ def perturbed(prob, seed=1, amp=1e-3):
         w[iw][ms] += amp * base * rng.standard_normal(w[iw][ms].shape)
     prob.w = w
     return prob
+
+
+def ulp_perturbed(prob, seed):
+    """prob.w with every evolving mesh value moved by -1, 0 or +1 ulp (drawn
+    per value from `seed`): a start that differs from prob.w by rounding alone"""
+    ne = problems.var_index(prob.ndir)["nevolve"]
+    w = prob.w.copy()
+    ms = (slice(0, ne),) + prob.mesh_slices()
+    m = w[ms]
+    k = np.random.default_rng(seed).integers(-1, 2, size=m.shape)
+    w[ms] = np.where(k > 0, np.nextafter(m, np.inf), np.where(k < 0, np.nextafter(m, -np.inf), m))
+    return w
