@@ -62,7 +62,7 @@ extern "C" {
 #define SAC_SLOT_W 0  /* the state w */
 #define SAC_SLOT_S1 1 /* stage registers */
 #define SAC_SLOT_S2 2
-#define SAC_SLOT_ACC 3
+#define SAC_SLOT_S3 3 /* RK4's third stage register */
 
 /* kernels reported by sac_dev_timing_get */
 #define SAC_K_FINAL 0 /* cross terms, gravity, register update, cmax reductions */

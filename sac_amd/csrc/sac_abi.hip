@@ -247,8 +247,10 @@ void cmax_reduce(Dev& d, int slot) {
   d.red_valid[slot] = true;
 }
 
-// one stage: out = base + qa*R(wct) ; acc_out = acc_in + qb*R ; ghosts of out
-void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_out, double qb) {
+// one stage: out = base + qa*R(wct), ghosts of out.  rk4_last: RK4's last
+// stage, out also gets (S1 + 2*S2 + S3 - 4*base)/3 of the three earlier
+// stage registers (the reference's w1, w2, w3)
+void stage(Dev& d, int wct, int base, int out, double qa, bool rk4_last = false) {
   if (!d.red_valid[wct]) cmax_reduce(d, wct);
   sac::AxisArgs x{};
   x.g = d.g;
@@ -323,10 +325,12 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   s.T = d.cross;
   s.base = d.reg[base];
   s.out = d.reg[out];
-  s.acc_in = acc_in >= 0 ? d.reg[acc_in] : nullptr;
-  s.acc_out = acc_out >= 0 ? d.reg[acc_out] : nullptr;
+  if (rk4_last) {
+    s.w1 = d.reg[SAC_SLOT_S1];
+    s.w2 = d.reg[SAC_SLOT_S2];
+    s.w3 = d.reg[SAC_SLOT_S3];
+  }
   s.qa = qa;
-  s.qb = qb;
   s.gamma = d.gamma;
   for (int k = 0; k < 3; k++) {
     s.grav[k] = d.grav[k];
@@ -342,7 +346,6 @@ void stage(Dev& d, int wct, int base, int out, double qa, int acc_in, int acc_ou
   sac::launch_final(s, d.stream);
   t_end(d, SAC_K_FINAL, e0);
   d.red_valid[out] = true;
-  if (acc_out >= 0) d.red_valid[acc_out] = false;
   if (d.any_zero)
     ghost_fill_sequential(d, out, false);
   else
@@ -882,50 +885,51 @@ int sac_dev_advance(sac_handle h, double dt) {
   Dev& d = *(Dev*)h;
   if (!d.initialized) return fail(SAC_E_ARG, "getboundary must run before advance");
   if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SAC_E_ARG, "dt=%g", dt);
-  const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, A = SAC_SLOT_ACC;
+  const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, S3 = SAC_SLOT_S3;
   // The integrators of src/vac.t:406-475.  A stage is
-  // out = base + qa*R(wct) (acc_out = acc_in + qb*R), followed by the ghost
-  // fill of out, as advect1 ends with getboundary (src/vac.t:544).
+  // out = base + qa*R(wct), followed by the ghost fill of out, as advect1
+  // ends with getboundary (src/vac.t:544).
   const size_t nreg = (size_t)d.ne * d.g.fs;
   switch (d.typeadvance) {
     case SAC_ADV_FOURSTEP:
-      // classical RK4 (:439-455) in two registers plus accumulator
-      stage(d, W, W, S1, dt / 2, W, A, dt / 6);
-      stage(d, S1, W, S2, dt / 2, A, A, dt / 3);
-      stage(d, S2, W, S1, dt, A, A, dt / 3);
-      stage(d, S1, A, W, dt / 6, -1, -1, 0.0);
+      // classical RK4 (:439-455): w1, w2, w3 in S1, S2, S3; the last stage
+      // adds (w1 + 2*w2 + w3 - 4*w)/3 to w + dt/6*k4 (:450-454) as it forms it
+      stage(d, W, W, S1, dt / 2);
+      stage(d, S1, W, S2, dt / 2);
+      stage(d, S2, W, S3, dt);
+      stage(d, S3, W, W, dt / 6, true);
       break;
     case SAC_ADV_TWOSTEP:
       // predictor advect1(dt/2, w -> w1), full step advect1(dt, wCT=w1, w) (:424-428)
-      stage(d, W, W, S1, dt / 2, -1, -1, 0.0);
-      stage(d, S1, W, W, dt, -1, -1, 0.0);
+      stage(d, W, W, S1, dt / 2);
+      stage(d, S1, W, W, dt);
       break;
     case SAC_ADV_THREESTEP:
       // Shu's third-order method (:430-436): w1 = w + dt R(w);
       // w2 = 3/4 w + 1/4 w1 + dt/4 R(w1); w = w/3 + 2/3 w2 + 2dt/3 R(w2)
-      stage(d, W, W, S1, dt, -1, -1, 0.0);
+      stage(d, W, W, S1, dt);
       sac::launch_axpy4(d.prec, (long long)nreg, d.reg[S2], d.reg[W], 0.75, d.reg[S1], 0.25,
                         nullptr, 0.0, nullptr, 0.0, d.stream);
       d.red_valid[S2] = false;
-      stage(d, S1, S2, S2, dt / 4, -1, -1, 0.0);
+      stage(d, S1, S2, S2, dt / 4);
       sac::launch_axpy4(d.prec, (long long)nreg, d.reg[W], d.reg[W], 1.0 / 3.0, d.reg[S2],
                         2.0 / 3.0, nullptr, 0.0, nullptr, 0.0, d.stream);
       d.red_valid[W] = false;
-      stage(d, S2, W, W, dt * 2.0 / 3.0, -1, -1, 0.0);
+      stage(d, S2, W, W, dt * 2.0 / 3.0);
       break;
     case SAC_ADV_STERCK:
       // (:458-466); 0.12 is a default-real (single precision) literal there
-      stage(d, W, W, S1, dt * (double)0.12f, -1, -1, 0.0);
-      stage(d, S1, W, S2, dt / 4, -1, -1, 0.0);
-      stage(d, S2, W, S1, dt / 2, -1, -1, 0.0);
-      stage(d, S1, W, W, dt, -1, -1, 0.0);
+      stage(d, W, W, S1, dt * (double)0.12f);
+      stage(d, S1, W, S2, dt / 4);
+      stage(d, S2, W, S1, dt / 2);
+      stage(d, S1, W, W, dt);
       break;
     case SAC_ADV_JAMESON:
       // (:468-475)
-      stage(d, W, W, S1, dt / 4, -1, -1, 0.0);
-      stage(d, S1, W, S2, dt / 3, -1, -1, 0.0);
-      stage(d, S2, W, S1, dt / 2, -1, -1, 0.0);
-      stage(d, S1, W, W, dt, -1, -1, 0.0);
+      stage(d, W, W, S1, dt / 4);
+      stage(d, S1, W, S2, dt / 3);
+      stage(d, S2, W, S1, dt / 2);
+      stage(d, S1, W, W, dt);
       break;
     default: {
       // onestep: w1=w; advect1(dt, wCT=w1, w) (:407-408)
@@ -936,7 +940,7 @@ int sac_dev_advance(sac_handle h, double dt) {
         HIPCHK(hipMemcpyAsync(red_of(d, S1), red_of(d, W),
                               sizeof(unsigned long long) * sac::kRedSlots,
                               hipMemcpyDeviceToDevice, d.stream));
-      stage(d, S1, W, W, dt, -1, -1, 0.0);
+      stage(d, S1, W, W, dt);
     }
   }
   HIPCHK(hipGetLastError());
@@ -975,7 +979,7 @@ int sac_dev_advect1(sac_handle h, double qdt, int wct_slot, int w_slot) {
   if (wct_slot < 0 || wct_slot >= kSlots || w_slot < 0 || w_slot >= kSlots || wct_slot == w_slot)
     return fail(SAC_E_ARG, "advect1 needs two distinct slots (%d,%d)", wct_slot, w_slot);
   if (!d.initialized) return fail(SAC_E_ARG, "getboundary must run before advect1");
-  stage(d, wct_slot, w_slot, w_slot, qdt, -1, -1, 0.0);
+  stage(d, wct_slot, w_slot, w_slot, qdt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(d.stream));
   return 0;

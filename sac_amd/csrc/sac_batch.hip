@@ -194,12 +194,22 @@ __device__ __forceinline__ void cont_ghosts_of(const GridDesc& g, double* f, con
   }
 }
 
-// Patches: combining update of one tile: out = base + dt/da * R, acc_out =
-// acc_in + dt/db * R, with the x derivatives of the cross factors, then the
-// 'cont' ghosts of out
+// The three earlier stage registers of RK4's last stage (rk4_last), as in
+// the grid path: out = (base + dt/da*R) + (S1 + 2*S2 + S3 - 4*base)/3
+__device__ __forceinline__ void set_rk4(const Args& a, long long o, int rk4_last, FinalArgs& f) {
+  if (rk4_last) {
+    f.w1 = a.reg[SAC_SLOT_S1] + o;
+    f.w2 = a.reg[SAC_SLOT_S2] + o;
+    f.w3 = a.reg[SAC_SLOT_S3] + o;
+  }
+}
+
+// Patches: combining update of one tile: out = base + dt/da * R (see
+// set_rk4), with the x derivatives of the cross factors, then the 'cont'
+// ghosts of out
 template <int NDIM, int CK>
 __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int out, double da,
-                                                int acc_in, int acc_out, double db) {
+                                                int rk4_last) {
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;
   const GridDesc g = tube_grid<NDIM>(a, tl.tube);
@@ -216,10 +226,8 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
   f.T = (NDIM == 2) ? a.T + o : nullptr;
   f.base = a.reg[base] + o;
   f.out = a.reg[out] + o;
-  f.acc_in = acc_in >= 0 ? a.reg[acc_in] + o : nullptr;
-  f.acc_out = acc_out >= 0 ? a.reg[acc_out] + o : nullptr;
+  set_rk4(a, o, rk4_last, f);
   f.qa = dt / da;
-  f.qb = acc_out >= 0 ? dt / db : 0.0;
   f.gamma = a.gamma;
   f.visc_on = a.visc_on;
   f.skip_last = (NDIM == 2) ? 1 : 0;  // the y line tiles added their cross terms
@@ -248,7 +256,7 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
 // been cleared by kb_zero_red.
 template <int LT>
 __global__ void __launch_bounds__(LT) kb_tube_stage(Args a, int wct, int base, int out, double da,
-                                                   int acc_in, int acc_out, double db) {
+                                                   int rk4_last) {
   __shared__ double L[line1::LV<NDIR> * (LT + 6)];
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
@@ -268,10 +276,8 @@ __global__ void __launch_bounds__(LT) kb_tube_stage(Args a, int wct, int base, i
   f.R = a.R + o;
   f.base = a.reg[base] + o;
   f.out = a.reg[out] + o;
-  f.acc_in = acc_in >= 0 ? a.reg[acc_in] + o : nullptr;
-  f.acc_out = acc_out >= 0 ? a.reg[acc_out] + o : nullptr;
+  set_rk4(a, o, rk4_last, f);
   f.qa = dt / da;
-  f.qb = acc_out >= 0 ? dt / db : 0.0;
   f.gamma = a.gamma;
   f.visc_on = a.visc_on;
   double redv[2] = {0.0, 0.0};
@@ -600,33 +606,32 @@ void enqueue_step_ck(Batch& b) {
   hipStream_t st = b.stream;
   const dim3 gt(b.ntiles), bt(b.shape.tb);
   const dim3 gn((b.ntubes + 255) / 256), bn(256);
-  const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, A = SAC_SLOT_ACC;
+  const int W = SAC_SLOT_W, S1 = SAC_SLOT_S1, S2 = SAC_SLOT_S2, S3 = SAC_SLOT_S3;
   hipLaunchKernelGGL(kb_dt, gn, bn, 0, st, a);
-  // (wct, base, out, da, acc_in, acc_out, db) of the four RK4 stages, as in
-  // the grid path: S1=W+dt/2 k1, A=W+dt/6 k1; S2=W+dt/2 k2, A+=dt/3 k2;
-  // S1=W+dt k3, A+=dt/3 k3; W=A+dt/6 k4
+  // (wct, base, out, da, rk4_last) of the four RK4 stages, as in the grid
+  // path: S1=W+dt/2 k1; S2=W+dt/2 k2; S3=W+dt k3;
+  // W=(W+dt/6 k4)+(S1+2 S2+S3-4 W)/3
   struct St {
     int wct, base, out;
     double da;
-    int ai, ao;
-    double db;
+    int last;
   };
-  const St stages[4] = {{W, W, S1, 2.0, W, A, 6.0},
-                        {S1, W, S2, 2.0, A, A, 3.0},
-                        {S2, W, S1, 1.0, A, A, 3.0},
-                        {S1, A, W, 6.0, -1, -1, 0.0}};
+  const St stages[4] = {{W, W, S1, 2.0, 0},
+                        {S1, W, S2, 2.0, 0},
+                        {S2, W, S3, 1.0, 0},
+                        {S3, W, W, 6.0, 1}};
   for (const St& s : stages) {
     if constexpr (NDIM == 1) {
       hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, st, a, s.out);
       if (b.shape.tb == 64)
-        hipLaunchKernelGGL(kb_tube_stage<64>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
-                           s.ao, s.db);
+        hipLaunchKernelGGL(kb_tube_stage<64>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
+                           s.last);
       else if (b.shape.tb == 128)
-        hipLaunchKernelGGL(kb_tube_stage<128>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
-                           s.ao, s.db);
+        hipLaunchKernelGGL(kb_tube_stage<128>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
+                           s.last);
       else
-        hipLaunchKernelGGL(kb_tube_stage<256>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
-                           s.ao, s.db);
+        hipLaunchKernelGGL(kb_tube_stage<256>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
+                           s.last);
       continue;
     } else {
       hipLaunchKernelGGL(kb_pline<0>, dim3(a.nltiles[0]), dim3(sac::line::Tile<0, double>::T), 0, st,
@@ -634,8 +639,8 @@ void enqueue_step_ck(Batch& b) {
       hipLaunchKernelGGL(kb_pline<1>, dim3(a.nltiles[1]), dim3(sac::line::Tile<1, double>::T), 0, st,
                          a, s.wct, s.out);
     }
-    hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da, s.ai,
-                       s.ao, s.db);
+    hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
+                       s.last);
   }
   hipLaunchKernelGGL(kb_tick, gn, bn, 0, st, a);
 }

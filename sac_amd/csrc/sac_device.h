@@ -682,8 +682,9 @@ __device__ __forceinline__ void line1_tile(const GridDesc& g, const double* __re
 
 // ---------------------------------------------------------------------
 // kernel 2: gravity, derivatives of the cross-term factors, the register
-// update out = base + qa*R (acc_out = acc_in + qb*R) and the cmax
-// reductions of the result.
+// update out = base + qa*R (in RK4's last stage plus the reference's
+// (w1 + 2*w2 + w3 - 4*w)/3, src/vac.t:450-454) and the cmax reductions of
+// the result.
 //   m_k += d_l(rho_t Tm_{k,l}),  e += d_l(m_k Tm_{k,l})
 //   b_k -= d_l(Tb_{k,l}),        e -= d_l(b_k Tb_{k,l})
 // with central differences along l, zero at a non-periodic edge of l, and
@@ -695,8 +696,9 @@ __device__ __forceinline__ void line1_tile(const GridDesc& g, const double* __re
 // cells and dropped); the max of the cmax values of the stored results goes
 // to redv.  Each statement handles the V cells together, so that the loads
 // of neighbouring cells (FP32 storage: 4 bytes apart) can be issued as one
-// wider load; the arithmetic of every cell is that of V = 1.
-template <int NDIM, int NDIR, typename S, typename C, int V>
+// wider load; the arithmetic of every cell is that of V = 1.  RK4L: RK4's
+// last stage (a.w1, a.w2, a.w3 set).
+template <int NDIM, int NDIR, typename S, typename C, int V, bool RK4L>
 __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& g,
                                             const int (&c)[3], int nv,
                                             double (&redv)[2 * NDIM]) {
@@ -805,53 +807,63 @@ __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& 
   S* __restrict__ out = (S*)a.out;
   const S* __restrict__ base = (const S*)a.base;
   S o[V][NE];
+  // RK4's last stage: the reference's w1 = (w1 + 2*w2 + w3 - 4*w)/3 of the
+  // three earlier stage registers and the step's base w, added to the update
+  // (its left-to-right order kept, no contraction)
+  const S* __restrict__ r1 = (const S*)a.w1;
+  const S* __restrict__ r2 = (const S*)a.w2;
+  const S* __restrict__ r3 = (const S*)a.w3;
+  auto rk4sum = [&](long long i, double b) {
+#pragma clang fp contract(off)
+    return ((((double)r1[i] + 2.0 * (double)r2[i]) + (double)r3[i]) - 4.0 * b) / 3.0;
+  };
   if constexpr (V == 1) {
     // each field stored as soon as it is formed: measured faster at FP64
     // (1.46 against 1.48 ms per 256^3 launch) than all loads before the stores
+    if constexpr (RK4L) {
 #pragma unroll
-    for (int n = 0; n < NE; n++) {
-      o[0][n] = (S)((double)base[n * fs + ic0] + a.qa * (double)R[0][n]);
-      out[n * fs + ic0] = o[0][n];
-    }
-    if (a.acc_out) {
-      S* __restrict__ ao = (S*)a.acc_out;
-      const S* __restrict__ ai = (const S*)a.acc_in;
+      for (int n = 0; n < NE; n++) {
+        const double b = (double)base[n * fs + ic0];
+        const double u = b + a.qa * (double)R[0][n];
+        o[0][n] = (S)(u + rk4sum(n * fs + ic0, b));
+        out[n * fs + ic0] = o[0][n];
+      }
+    } else {
 #pragma unroll
-      for (int n = 0; n < NE; n++)
-        ao[n * fs + ic0] = (S)((double)ai[n * fs + ic0] + a.qb * (double)R[0][n]);
+      for (int n = 0; n < NE; n++) {
+        o[0][n] = (S)((double)base[n * fs + ic0] + a.qa * (double)R[0][n]);
+        out[n * fs + ic0] = o[0][n];
+      }
     }
   } else {
+    if constexpr (RK4L) {
 #pragma unroll
-    for (int n = 0; n < NE; n++)
+      for (int n = 0; n < NE; n++)
 #pragma unroll
-      for (int j = 0; j < V; j++)
-        o[j][n] = (S)((double)base[n * fs + ic0 + j] + a.qa * (double)R[j][n]);
-    // all V cells stored in one branch (paired stores), the first nv in the other
-    auto store = [&](S* __restrict__ dst, const S (&val)[V][NE]) {
-      if (nv == V) {
-#pragma unroll
-        for (int n = 0; n < NE; n++)
-#pragma unroll
-          for (int j = 0; j < V; j++) dst[n * fs + ic0 + j] = val[j][n];
-      } else {
-#pragma unroll
-        for (int n = 0; n < NE; n++)
-#pragma unroll
-          for (int j = 0; j < V; j++)
-            if (j < nv) dst[n * fs + ic0 + j] = val[j][n];
-      }
-    };
-    store(out, o);
-    if (a.acc_out) {
-      S* __restrict__ ao = (S*)a.acc_out;
-      const S* __restrict__ ai = (const S*)a.acc_in;
-      S q[V][NE];
+        for (int j = 0; j < V; j++) {
+          const double b = (double)base[n * fs + ic0 + j];
+          const double u = b + a.qa * (double)R[j][n];
+          o[j][n] = (S)(u + rk4sum(n * fs + ic0 + j, b));
+        }
+    } else {
 #pragma unroll
       for (int n = 0; n < NE; n++)
 #pragma unroll
         for (int j = 0; j < V; j++)
-          q[j][n] = (S)((double)ai[n * fs + ic0 + j] + a.qb * (double)R[j][n]);
-      store(ao, q);
+          o[j][n] = (S)((double)base[n * fs + ic0 + j] + a.qa * (double)R[j][n]);
+    }
+    // all V cells stored in one branch (paired stores), the first nv in the other
+    if (nv == V) {
+#pragma unroll
+      for (int n = 0; n < NE; n++)
+#pragma unroll
+        for (int j = 0; j < V; j++) out[n * fs + ic0 + j] = o[j][n];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NE; n++)
+#pragma unroll
+        for (int j = 0; j < V; j++)
+          if (j < nv) out[n * fs + ic0 + j] = o[j][n];
     }
   }
 #pragma unroll
@@ -887,7 +899,10 @@ __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& 
 template <int NDIM, int NDIR, typename S, typename C>
 __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g, const int (&c)[3],
                                            double (&redv)[2 * NDIM]) {
-  final_cells<NDIM, NDIR, S, C, 1>(a, g, c, 1, redv);
+  if (a.w1)
+    final_cells<NDIM, NDIR, S, C, 1, true>(a, g, c, 1, redv);
+  else
+    final_cells<NDIM, NDIR, S, C, 1, false>(a, g, c, 1, redv);
 }
 
 // ---------------------------------------------------------------------

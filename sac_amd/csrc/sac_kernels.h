@@ -35,9 +35,11 @@ struct FinalArgs {
   const void* T;
   const void* base;      // out = base + qa*R
   void* out;
-  const void* acc_in;    // acc_out = acc_in + qb*R (acc_out may be null)
-  void* acc_out;
-  double qa, qb;
+  // RK4's last stage (w1 not null): out = (base + qa*R) + (w1 + 2*w2 + w3 - 4*base)/3
+  const void* w1;
+  const void* w2;
+  const void* w3;
+  double qa;
   double gamma;
   double grav[3];
   int grav_on[3];

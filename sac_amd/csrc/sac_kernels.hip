@@ -137,7 +137,7 @@ static void launch_line(const AxisArgs& A, hipStream_t st) {
 }
 
 // kernel 2: the combining update of every mesh cell (see final_cells), FV
-// consecutive cells along x per thread
+// consecutive cells along x per thread; RK4L: RK4's last stage
 template <int NDIM, int NDIR, typename S, typename C>
 struct FinalV {
   // FP32 storage: two cells per thread, so that neighbouring cells' loads
@@ -145,7 +145,7 @@ struct FinalV {
   static constexpr int V = (sizeof(S) == 4) ? 2 : 1;
 };
 
-template <int NDIM, int NDIR, typename S, typename C>
+template <int NDIM, int NDIR, typename S, typename C, bool RK4L>
 __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
   constexpr int V = FinalV<NDIM, NDIR, S, C>::V;
   const GridDesc& g = a.g;
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a) {
     decode_box(t, lo, ext, c);
     c[0] = lo[0] + V * (c[0] - lo[0]);
     const int nv = (V == 1) ? 1 : min(V, g.Mhi[0] - c[0] + 1);
-    final_cells<NDIM, NDIR, S, C, V>(a, g, c, nv, redv);
+    final_cells<NDIM, NDIR, S, C, V, RK4L>(a, g, c, nv, redv);
   }
   block_max_atomic<2 * NDIM>(redv, a.red_out);
 }
@@ -456,8 +456,11 @@ static void launch_nd(int which, int axis, const void* args, hipStream_t st, lon
       if (axis == 2)
         hipLaunchKernelGGL((k_axis<NDIM, NDIR, 2, S, C>), dim3(nb), dim3(bs), 0, st, A);
   } else if (which == 1) {
-    hipLaunchKernelGGL((k_final<NDIM, NDIR, S, C>), dim3(nb), dim3(bs), 0, st,
-                       *(const FinalArgs*)args);
+    const FinalArgs& f = *(const FinalArgs*)args;
+    if (f.w1)
+      hipLaunchKernelGGL((k_final<NDIM, NDIR, S, C, true>), dim3(nb), dim3(bs), 0, st, f);
+    else
+      hipLaunchKernelGGL((k_final<NDIM, NDIR, S, C, false>), dim3(nb), dim3(bs), 0, st, f);
   } else {
     hipLaunchKernelGGL((k_cmax<NDIM, NDIR, S>), dim3(nb), dim3(bs), 0, st,
                        *(const CmaxArgs*)args);

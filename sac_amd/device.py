@@ -16,7 +16,7 @@ from . import problems
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsac_amd.so")
 
-SLOT_W, SLOT_S1, SLOT_S2, SLOT_ACC = 0, 1, 2, 3
+SLOT_W, SLOT_S1, SLOT_S2, SLOT_S3 = 0, 1, 2, 3
 K_FINAL, K_AXIS, K_GHOST, K_CMAX, K_COPY = 0, 1, 2, 3, 4
 K_AXIS1, K_AXIS2, K_AXIS3 = 5, 6, 7
 K_CLOSURE, K_PLANE = 8, 9

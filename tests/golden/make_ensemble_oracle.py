@@ -42,9 +42,13 @@ def _members(which):
 
 def run_member(args):
     which, i = args
+    return (i,) + main_loop(_members_cache[which][i])
+
+
+def main_loop(prob):
+    """the reference's main loop on one member -> (steps, t, end state)"""
     from oracle import oracle as om
     from tests.helpers import make_oracle
-    prob = _members_cache[which][i]
     o = make_oracle(prob, om)
     t, it = 0.0, 0
     while True:
@@ -63,7 +67,7 @@ def run_member(args):
         it += 1
         t += dt
         o.set_time(it, 0, t)
-    return i, it, t, state
+    return it, t, state
 
 
 _members_cache = {}

@@ -413,9 +413,11 @@ def test_ensemble_endings_match_oracle(mods, which):
     member takes to t=0.1 is compared loosely: the viscous limit of a shock
     tube's dt is a ratio of rounding-level differences in its flat regions
     (tests/test_dt_conditioning.py), so the dt sequences of any two
-    implementations part company and the step counts differ (measured: 1746
-    of 4074 finished tubes differ, by at most 9.7 %; 425 of 1018 patches, by
-    at most 22 %; the median difference is 0 for both)."""
+    implementations part company and the step counts differ (measured: 1674
+    of 4074 finished tubes differ, by at most 10.9 %; 465 of 1017 patches,
+    by at most 13.4 %; the median difference is 0 for both).  Members that the
+    oracle's own rounding variants end differently (a near blow-up that
+    finishes in one FP64 run and stops in another) are exempt."""
     import os
     batch, _ = mods
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
@@ -433,7 +435,25 @@ def test_ensemble_endings_match_oracle(mods, which):
     stopped_ref = set(np.nonzero(ref["state"] == 3)[0].tolist())
     print(f"{which}: stopped at dt < dtmin (member: GPU step, oracle step) " +
           ", ".join(f"{i}: {it[i]}, {ref['it'][i]}" for i in sorted(stopped | stopped_ref)))
+    # members whose ending the oracle's own rounding variants do not agree on
+    # (ensemble_{which}_marginal.npz, make_ensemble_marginal.py): either
+    # ending is accepted for them
+    marginal = set()
+    mpath = os.path.join(os.path.dirname(path), f"ensemble_{which}_marginal.npz")
+    if os.path.exists(mpath):
+        mg = np.load(mpath)
+        for k, i in enumerate(mg["members"].tolist()):
+            if len(set(mg["state"][k].tolist()) | {int(ref["state"][i])}) > 1:
+                marginal.add(i)
+                print(f"{which}: member {i} is rounding-marginal: the oracle ends it with "
+                      f"state {int(ref['state'][i])} after {int(ref['it'][i])} steps, its "
+                      "variants " + ", ".join(f"{v}: state {s_} after {n}" for v, s_, n in
+                                              zip(mg["variants"], mg["state"][k], mg["it"][k])) +
+                      f"; this run: state {int(st[i])} after {int(it[i])}")
+    stopped -= marginal
+    stopped_ref -= marginal
     ok = (st == 1) & (ref["state"] == 1)
+    ok[list(marginal)] = False
     rel = np.abs(it[ok] - ref["it"][ok]) / ref["it"][ok]
     print(f"{which}: steps of finished members: max rel. difference {rel.max():.2e}, median "
           f"{np.median(rel):.2e}, {int((it[ok] != ref['it'][ok]).sum())} of {int(ok.sum())} differ")
