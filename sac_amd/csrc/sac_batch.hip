@@ -41,7 +41,7 @@ constexpr int NDIR = 3;
 constexpr int NE = 2 + 2 * NDIR;  // evolving fields
 constexpr int NBG = 2 + NDIR;     // background fields
 constexpr int NT2 = 4;            // cross-term factor fields of a 2-D patch (2*ndim*(ndim-1))
-constexpr int kSlots = 4;         // W, S1, S2, ACC
+constexpr int kSlots = 4;         // W, S1, S2, S3
 constexpr int kRed = 4;           // per tube and register: max cmax_d, then max cmax_d/dx_d (d < ndim)
 
 // FINISHED: t >= tmax or it >= itmax; STOPPED: dt < dtmin (the reference's
