@@ -49,3 +49,34 @@ def ulp_perturbed(prob, seed):
     k = np.random.default_rng(seed).integers(-1, 2, size=m.shape)
     w[ms] = np.where(k > 0, np.nextafter(m, np.inf), np.where(k < 0, np.nextafter(m, -np.inf), m))
     return w
+
+
+def zero_boundary_problem(ndim):
+    """A problem with 'zero' boundaries (src/vacgrid.t:294-345) in 1-D (a
+    Riemann tube, zero on the left), 2-D (a Riemann patch, zero along x) or
+    3-D (the stratified box, zero along x and z)."""
+    if ndim == 1:
+        prob = problems.riemann_tube(40, {"rho": 1.0, "p": 0.8, "v": [0.1, -0.2, 0.3],
+                                          "b": [0.6, 0.5, -0.4]},
+                                     {"rho": 0.3, "p": 0.2, "v": [-0.1, 0.2, 0.0],
+                                      "b": [0.6, -0.5, 0.4]})
+        prob.typeB[:, :] = [problems.BC["zero"], problems.BC["cont"]]
+        prob.typeB[1, 1] = problems.BC["zero"]
+    elif ndim == 2:
+        prob = problems.riemann_patch(14, 12, {"rho": 1.0, "p": 0.8, "v": [0.1, -0.2, 0.3],
+                                              "b": [0.6, 0.5, -0.4]},
+                                      {"rho": 0.3, "p": 0.2, "v": [-0.1, 0.2, 0.0],
+                                       "b": [0.6, 0.5, 0.4]})
+        prob.typeB[:, :] = [problems.BC[c] for c in ("zero", "zero", "periodic", "periodic")]
+    else:
+        # the background's ghosts given with the state (as an ini file of the
+        # whole grid gives them): with zero background ghosts the first
+        # fill's conversions set the momentum of the ghosts filled before the
+        # background's to zero, and the stratified run blows up on both sides
+        prob = problems.stratified(n=10, nxyz=(10, 9, 8), amp=3e-2)
+        ms = prob.mesh_slices()
+        for iw in range(8, 13):
+            prob.w[iw] = np.pad(prob.w[iw][ms], 2, mode="edge")
+        prob.typeB[:, :] = [problems.BC[c] for c in ("zero", "cont", "periodic", "periodic",
+                                                    "zero", "zero")]
+    return prob

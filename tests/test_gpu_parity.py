@@ -9,7 +9,7 @@ import pytest
 
 from sac_amd import problems
 from tests.helpers import (assert_close_state, make_oracle, mesh, oracle_run,
-                           perturbed, per_var_errors)
+                           perturbed, per_var_errors, zero_boundary_problem)
 
 pytestmark = pytest.mark.gpu
 
@@ -369,30 +369,7 @@ def test_zero_boundary_matches_oracle(dev_mod, oracle_mod, ndim):
     in the ghosts not yet filled, as the reference's do) and a refill match
     the oracle bit for bit, NaN included; then a few RK4 steps under the
     oracle's dt sequence match to 1e-12."""
-    if ndim == 1:
-        prob = problems.riemann_tube(40, {"rho": 1.0, "p": 0.8, "v": [0.1, -0.2, 0.3],
-                                          "b": [0.6, 0.5, -0.4]},
-                                     {"rho": 0.3, "p": 0.2, "v": [-0.1, 0.2, 0.0],
-                                      "b": [0.6, -0.5, 0.4]})
-        prob.typeB[:, :] = [problems.BC["zero"], problems.BC["cont"]]
-        prob.typeB[1, 1] = problems.BC["zero"]
-    elif ndim == 2:
-        prob = problems.riemann_patch(14, 12, {"rho": 1.0, "p": 0.8, "v": [0.1, -0.2, 0.3],
-                                              "b": [0.6, 0.5, -0.4]},
-                                      {"rho": 0.3, "p": 0.2, "v": [-0.1, 0.2, 0.0],
-                                       "b": [0.6, 0.5, 0.4]})
-        prob.typeB[:, :] = [problems.BC[c] for c in ("zero", "zero", "periodic", "periodic")]
-    else:
-        # the background's ghosts given with the state (as an ini file of the
-        # whole grid gives them): with zero background ghosts the first
-        # fill's conversions set the momentum of the ghosts filled before the
-        # background's to zero, and the stratified run blows up on both sides
-        prob = problems.stratified(n=10, nxyz=(10, 9, 8), amp=3e-2)
-        ms = prob.mesh_slices()
-        for iw in range(8, 13):
-            prob.w[iw] = np.pad(prob.w[iw][ms], 2, mode="edge")
-        prob.typeB[:, :] = [problems.BC[c] for c in ("zero", "cont", "periodic", "periodic",
-                                                    "zero", "zero")]
+    prob = zero_boundary_problem(ndim)
     o = make_oracle(prob, oracle_mod)
     ref = o.get_w()
     dev = dev_mod.Device(prob)

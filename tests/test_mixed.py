@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 from sac_amd import problems
-from tests.helpers import make_oracle, oracle_run, per_var_errors, perturbed
+from tests.helpers import (make_oracle, oracle_run, per_var_errors, perturbed,
+                           zero_boundary_problem)
 
 pytestmark = pytest.mark.gpu
 
@@ -110,6 +111,24 @@ def test_mixed_brio_wu_prescribed_dt(dev_mod, oracle_mod):
     dev.close()
     assert np.isfinite(w).all()
     errs = per_var_errors(prob, w, ref, 6)
+    assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("ndim", [1, 3])
+def test_mixed_zero_boundary(dev_mod, oracle_mod, ndim):
+    """'zero' boundaries in mixed precision (the primitive/conserve round
+    trips of each fill on the FP32 register, in FP64 arithmetic): three RK4
+    steps under the oracle's dt sequence at FP32 resolution"""
+    prob = zero_boundary_problem(ndim)
+    ref, rdts, _ = oracle_run(prob, oracle_mod, 3)
+    dev = dev_mod.Device(prob, precision="mixed")
+    dev.upload(prob.w)
+    dev.getboundary()
+    for dt in rdts:
+        dev.advance(dt)
+    w = dev.download()
+    dev.close()
+    errs = per_var_errors(prob, w, ref, 8)
     assert max(errs) <= 1e-5, errs
 
 
