@@ -194,10 +194,11 @@ __device__ __forceinline__ void cont_ghosts_of(const GridDesc& g, double* f, con
   }
 }
 
-// The three earlier stage registers of RK4's last stage (rk4_last), as in
-// the grid path: out = (base + dt/da*R) + (S1 + 2*S2 + S3 - 4*base)/3
-__device__ __forceinline__ void set_rk4(const Args& a, long long o, int rk4_last, FinalArgs& f) {
-  if (rk4_last) {
+// The three earlier stage registers of RK4's last stage (RK4L), as in the
+// grid path: out = (base + dt/da*R) + (S1 + 2*S2 + S3 - 4*base)/3
+template <bool RK4L>
+__device__ __forceinline__ void set_rk4(const Args& a, long long o, FinalArgs& f) {
+  if constexpr (RK4L) {
     f.w1 = a.reg[SAC_SLOT_S1] + o;
     f.w2 = a.reg[SAC_SLOT_S2] + o;
     f.w3 = a.reg[SAC_SLOT_S3] + o;
@@ -207,9 +208,8 @@ __device__ __forceinline__ void set_rk4(const Args& a, long long o, int rk4_last
 // Patches: combining update of one tile: out = base + dt/da * R (see
 // set_rk4), with the x derivatives of the cross factors, then the 'cont'
 // ghosts of out
-template <int NDIM, int CK>
-__global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int out, double da,
-                                                int rk4_last) {
+template <int NDIM, int CK, bool RK4L>
+__global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int out, double da) {
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;
   const GridDesc g = tube_grid<NDIM>(a, tl.tube);
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
   f.T = (NDIM == 2) ? a.T + o : nullptr;
   f.base = a.reg[base] + o;
   f.out = a.reg[out] + o;
-  set_rk4(a, o, rk4_last, f);
+  set_rk4<RK4L>(a, o, f);
   f.qa = dt / da;
   f.gamma = a.gamma;
   f.visc_on = a.visc_on;
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
     int c[3];
     if (tile_cell<NDIM, CK>(a, tl, g, j, c)) {
       double r[2 * NDIM];
-      final_cell<NDIM, NDIR, double, double>(f, g, c, r);
+      final_cell<NDIM, NDIR, double, double, RK4L>(f, g, c, r);
       cont_ghosts_of<NDIM>(g, (double*)f.out, c);
 #pragma unroll
       for (int k = 0; k < 2 * NDIM; k++) redv[k] = fmax(redv[k], r[k]);
@@ -254,9 +254,8 @@ __global__ void __launch_bounds__(256) kb_final(Args a, int wct, int base, int o
 // the thread that stored R of its cell reads it back, so no other block is
 // waited for) with the 'cont' ghosts.  The reductions of `out` must have
 // been cleared by kb_zero_red.
-template <int LT>
-__global__ void __launch_bounds__(LT) kb_tube_stage(Args a, int wct, int base, int out, double da,
-                                                   int rk4_last) {
+template <int LT, bool RK4L>
+__global__ void __launch_bounds__(LT) kb_tube_stage(Args a, int wct, int base, int out, double da) {
   __shared__ double L[line1::LV<NDIR> * (LT + 6)];
   const Tile tl = a.tiles[blockIdx.x];
   if (a.state[tl.tube] != RUNNING) return;  // uniform over the block
@@ -276,14 +275,14 @@ __global__ void __launch_bounds__(LT) kb_tube_stage(Args a, int wct, int base, i
   f.R = a.R + o;
   f.base = a.reg[base] + o;
   f.out = a.reg[out] + o;
-  set_rk4(a, o, rk4_last, f);
+  set_rk4<RK4L>(a, o, f);
   f.qa = dt / da;
   f.gamma = a.gamma;
   f.visc_on = a.visc_on;
   double redv[2] = {0.0, 0.0};
   int c[3] = {tl.c0 + (int)threadIdx.x, 0, 0};
   if (c[0] <= g.Mhi[0]) {
-    final_cell<1, NDIR, double, double>(f, g, c, redv);
+    final_cell<1, NDIR, double, double, RK4L>(f, g, c, redv);
     cont_ghosts_of<1>(g, (double*)f.out, c);
   }
   if (a.visc_on) {
@@ -600,6 +599,18 @@ void enqueue_ghost(Batch& b, double* f, int nfield) {
   }
 }
 
+// one tube stage kernel at the batch's tile length
+template <bool RK4L>
+void launch_tube_stage(Batch& b, dim3 gt, dim3 bt, int wct, int base, int out, double da) {
+  const Args& a = b.a;
+  if (b.shape.tb == 64)
+    hipLaunchKernelGGL((kb_tube_stage<64, RK4L>), gt, bt, 0, b.stream, a, wct, base, out, da);
+  else if (b.shape.tb == 128)
+    hipLaunchKernelGGL((kb_tube_stage<128, RK4L>), gt, bt, 0, b.stream, a, wct, base, out, da);
+  else
+    hipLaunchKernelGGL((kb_tube_stage<256, RK4L>), gt, bt, 0, b.stream, a, wct, base, out, da);
+}
+
 template <int NDIM, int CK>
 void enqueue_step_ck(Batch& b) {
   const Args& a = b.a;
@@ -623,15 +634,8 @@ void enqueue_step_ck(Batch& b) {
   for (const St& s : stages) {
     if constexpr (NDIM == 1) {
       hipLaunchKernelGGL(kb_zero_red, gn, bn, 0, st, a, s.out);
-      if (b.shape.tb == 64)
-        hipLaunchKernelGGL(kb_tube_stage<64>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
-                           s.last);
-      else if (b.shape.tb == 128)
-        hipLaunchKernelGGL(kb_tube_stage<128>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
-                           s.last);
-      else
-        hipLaunchKernelGGL(kb_tube_stage<256>, gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
-                           s.last);
+      if (s.last) launch_tube_stage<true>(b, gt, bt, s.wct, s.base, s.out, s.da);
+      else launch_tube_stage<false>(b, gt, bt, s.wct, s.base, s.out, s.da);
       continue;
     } else {
       hipLaunchKernelGGL(kb_pline<0>, dim3(a.nltiles[0]), dim3(sac::line::Tile<0, double>::T), 0, st,
@@ -639,8 +643,11 @@ void enqueue_step_ck(Batch& b) {
       hipLaunchKernelGGL(kb_pline<1>, dim3(a.nltiles[1]), dim3(sac::line::Tile<1, double>::T), 0, st,
                          a, s.wct, s.out);
     }
-    hipLaunchKernelGGL((kb_final<NDIM, CK>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da,
-                       s.last);
+    if (s.last)
+      hipLaunchKernelGGL((kb_final<NDIM, CK, true>), gt, bt, 0, st, a, s.wct, s.base, s.out, s.da);
+    else
+      hipLaunchKernelGGL((kb_final<NDIM, CK, false>), gt, bt, 0, st, a, s.wct, s.base, s.out,
+                         s.da);
   }
   hipLaunchKernelGGL(kb_tick, gn, bn, 0, st, a);
 }

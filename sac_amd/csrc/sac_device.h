@@ -896,13 +896,11 @@ __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& 
 }
 
 // One mesh cell c of the grid g; the cmax values of the result go to redv.
-template <int NDIM, int NDIR, typename S, typename C>
+// RK4L: RK4's last stage (a.w1, a.w2, a.w3 set).
+template <int NDIM, int NDIR, typename S, typename C, bool RK4L>
 __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g, const int (&c)[3],
                                            double (&redv)[2 * NDIM]) {
-  if (a.w1)
-    final_cells<NDIM, NDIR, S, C, 1, true>(a, g, c, 1, redv);
-  else
-    final_cells<NDIM, NDIR, S, C, 1, false>(a, g, c, 1, redv);
+  final_cells<NDIM, NDIR, S, C, 1, RK4L>(a, g, c, 1, redv);
 }
 
 // ---------------------------------------------------------------------
