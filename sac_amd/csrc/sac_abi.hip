@@ -406,13 +406,12 @@ int copy_out(Dev& d, double* dst, const void* src, int nfield, size_t esz) {
   return 0;
 }
 
-double red_host(Dev& d, int slot, int k) {
-  unsigned long long v = 0;
-  hipMemcpyAsync(&v, red_of(d, slot) + k, sizeof(v), hipMemcpyDeviceToHost, d.stream);
+// the reductions [k0, k0+n) of register `slot` on the host (one copy, one wait)
+void red_host(Dev& d, int slot, int k0, int n, double* x) {
+  unsigned long long v[sac::kRedSlots] = {};
+  hipMemcpyAsync(v, red_of(d, slot) + k0, sizeof(v[0]) * n, hipMemcpyDeviceToHost, d.stream);
   hipStreamSynchronize(d.stream);
-  double x;
-  memcpy(&x, &v, sizeof(x));
-  return x;
+  memcpy(x, v, sizeof(double) * n);
 }
 
 }  // namespace
@@ -838,8 +837,10 @@ int sac_dev_getdt_courant(sac_handle h, double courantpar, double* dt, double* c
   Dev& d = *(Dev*)h;
   if (!d.red_valid[SAC_SLOT_W]) cmax_reduce(d, SAC_SLOT_W);
   HIPCHK(hipGetLastError());
+  double cmd[3] = {0.0, 0.0, 0.0};
+  red_host(d, SAC_SLOT_W, d.ndim, d.ndim, cmd);  // cmax_k/dx_k
   double cm = 0.0;
-  for (int k = 0; k < d.ndim; k++) cm = std::fmax(cm, red_host(d, SAC_SLOT_W, d.ndim + k));
+  for (int k = 0; k < d.ndim; k++) cm = std::fmax(cm, cmd[k]);
   double v = kBigDouble;
   if (cm > kSmallDouble) v = std::fmin(v, courantpar / cm);
   *dt = v;
