@@ -152,8 +152,9 @@ def timed_run(device, problems, n, steps, warmup, precision, calib_copy=False):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
-              "supported (multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the domain decomposition over several GPUs is a design, not built)",
+              file=sys.stderr)
         return 2
     from sac_amd import device, problems
     from sac_amd.errors import per_var_errors
