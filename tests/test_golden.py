@@ -68,3 +68,17 @@ def test_gpu_matches_fixture(name):
     print(f"{name}: per-variable rel-L2 vs fixture " + " ".join(f"{e:.2e}" for e in errs))
     bars = [max(tol, 4.0 * f) for f in g["floor"]]
     assert all(e <= b for e, b in zip(errs, bars)), (errs, bars)
+
+
+def test_marginal_members_end_both_ways():
+    """ensemble_patches_marginal.npz (make_ensemble_marginal.py) lists only
+    members whose endings differ between the oracle's plain run
+    (ensemble_patches_oracle.npz) and its rounding variants, so the endings
+    test exempts nothing that the oracle itself ends one way"""
+    mg = np.load(os.path.join(HERE, "ensemble_patches_marginal.npz"))
+    ref = np.load(os.path.join(HERE, "ensemble_patches_oracle.npz"))
+    assert list(mg["variants"]) == ["fma", "ulp1", "ulp2"]
+    assert mg["state"].shape == (len(mg["members"]), 3)
+    for k, i in enumerate(mg["members"].tolist()):
+        endings = set(mg["state"][k].tolist()) | {int(ref["state"][i])}
+        assert endings == {1, 3}, (i, endings)  # finished in one run, stopped in another
