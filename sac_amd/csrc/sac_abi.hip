@@ -248,8 +248,8 @@ void cmax_reduce(Dev& d, int slot) {
 }
 
 // one stage: out = base + qa*R(wct), ghosts of out.  rk4_last: RK4's last
-// stage, out also gets (S1 + 2*S2 + S3 - 4*base)/3 of the three earlier
-// stage registers (the reference's w1, w2, w3)
+// stage (wct is S3), out also gets (S1 + 2*S2 + S3 - 4*base)/3 of the three
+// earlier stage registers (the reference's w1, w2, w3)
 void stage(Dev& d, int wct, int base, int out, double qa, bool rk4_last = false) {
   if (!d.red_valid[wct]) cmax_reduce(d, wct);
   sac::AxisArgs x{};
@@ -328,7 +328,6 @@ void stage(Dev& d, int wct, int base, int out, double qa, bool rk4_last = false)
   if (rk4_last) {
     s.w1 = d.reg[SAC_SLOT_S1];
     s.w2 = d.reg[SAC_SLOT_S2];
-    s.w3 = d.reg[SAC_SLOT_S3];
   }
   s.qa = qa;
   s.gamma = d.gamma;

@@ -194,14 +194,14 @@ __device__ __forceinline__ void cont_ghosts_of(const GridDesc& g, double* f, con
   }
 }
 
-// The three earlier stage registers of RK4's last stage (RK4L), as in the
-// grid path: out = (base + dt/da*R) + (S1 + 2*S2 + S3 - 4*base)/3
+// The earlier stage registers of RK4's last stage (RK4L), as in the grid
+// path: out = (base + dt/da*R) + (S1 + 2*S2 + S3 - 4*base)/3, with S3 the
+// stage's input wct
 template <bool RK4L>
 __device__ __forceinline__ void set_rk4(const Args& a, long long o, FinalArgs& f) {
   if constexpr (RK4L) {
     f.w1 = a.reg[SAC_SLOT_S1] + o;
     f.w2 = a.reg[SAC_SLOT_S2] + o;
-    f.w3 = a.reg[SAC_SLOT_S3] + o;
   }
 }
 

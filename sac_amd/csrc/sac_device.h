@@ -697,7 +697,7 @@ __device__ __forceinline__ void line1_tile(const GridDesc& g, const double* __re
 // to redv.  Each statement handles the V cells together, so that the loads
 // of neighbouring cells (FP32 storage: 4 bytes apart) can be issued as one
 // wider load; the arithmetic of every cell is that of V = 1.  RK4L: RK4's
-// last stage (a.w1, a.w2, a.w3 set).
+// last stage (a.w1, a.w2 set; w3 is the stage input a.wct).
 template <int NDIM, int NDIR, typename S, typename C, int V, bool RK4L>
 __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& g,
                                             const int (&c)[3], int nv,
@@ -809,10 +809,11 @@ __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& 
   S o[V][NE];
   // RK4's last stage: the reference's w1 = (w1 + 2*w2 + w3 - 4*w)/3 of the
   // three earlier stage registers and the step's base w, added to the update
-  // (its left-to-right order kept, no contraction)
+  // (its left-to-right order kept, no contraction).  w3 is this stage's
+  // input, so its values at the cell are the loads above where they exist.
   const S* __restrict__ r1 = (const S*)a.w1;
   const S* __restrict__ r2 = (const S*)a.w2;
-  const S* __restrict__ r3 = (const S*)a.w3;
+  const S* __restrict__ r3 = W;
   auto rk4sum = [&](long long i, double b) {
 #pragma clang fp contract(off)
     return ((((double)r1[i] + 2.0 * (double)r2[i]) + (double)r3[i]) - 4.0 * b) / 3.0;
@@ -896,7 +897,7 @@ __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& 
 }
 
 // One mesh cell c of the grid g; the cmax values of the result go to redv.
-// RK4L: RK4's last stage (a.w1, a.w2, a.w3 set).
+// RK4L: RK4's last stage (a.w1, a.w2 set; w3 is the stage input a.wct).
 template <int NDIM, int NDIR, typename S, typename C, bool RK4L>
 __device__ __forceinline__ void final_cell(const FinalArgs& a, const GridDesc& g, const int (&c)[3],
                                            double (&redv)[2 * NDIM]) {

@@ -35,10 +35,10 @@ struct FinalArgs {
   const void* T;
   const void* base;      // out = base + qa*R
   void* out;
-  // RK4's last stage (w1 not null): out = (base + qa*R) + (w1 + 2*w2 + w3 - 4*base)/3
+  // RK4's last stage (w1 not null): out = (base + qa*R) + (w1 + 2*w2 + w3 - 4*base)/3,
+  // where w3 is the stage input wct
   const void* w1;
   const void* w2;
-  const void* w3;
   double qa;
   double gamma;
   double grav[3];
