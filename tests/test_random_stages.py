@@ -2,9 +2,10 @@
 update and maxviscoef) on seeded random grids against the oracle: random
 extents, random boundary pairs per axis, random perturbations, both 3-D
 schedules (plane tiles and one line tile per axis), 2-D patches and 1-D
-tubes.  Bar as for the fixed cases: increments rel-L2 <= 1e-11 per
-variable, states 1e-12, maxviscoef 1e-12; the 3-D cases in mixed precision
-at FP32 resolution (increments 1e-5, states 1e-6, maxviscoef 1e-5)."""
+tubes; and three RK4 steps on 8 more.  Bar as for the fixed cases:
+increments rel-L2 <= 1e-11 per variable, states 1e-12, maxviscoef 1e-12;
+the 3-D cases in mixed precision at FP32 resolution (increments 1e-5,
+states 1e-6, maxviscoef 1e-5)."""
 import numpy as np
 import pytest
 
@@ -90,3 +91,24 @@ def test_random_stage_mixed_matches_oracle(dev_mod, oracle_mod, seed):
     assert max(errs) <= 1e-5, f"{what}: increment rel-L2 {errs}"
     assert_close_state(prob, got, ref, 1e-6, 8, what)
     assert mv == pytest.approx(mv_ref, rel=1e-5), what
+
+
+@pytest.mark.parametrize("ndim,seed", [(3, s) for s in range(4)] + [(2, s) for s in range(2)] +
+                         [(1, s) for s in range(2)])
+def test_random_steps_match_oracle(dev_mod, oracle_mod, ndim, seed):
+    """three RK4 steps (the last stage's (w1 + 2 w2 + w3 - 4 w)/3 and every
+    ghost fill of the random boundary pairs) under the oracle's dt sequence"""
+    from tests.helpers import oracle_run
+    prob, codes = _problem(ndim, 7000 + 100 * ndim + seed)
+    ref, dts, _ = oracle_run(prob, oracle_mod, 3)
+    dev = dev_mod.Device(prob)
+    if ndim == 3:
+        dev.set_plane_tiles(bool(seed % 2))
+    dev.upload(prob.w)
+    dev.getboundary()
+    for dt in dts:
+        dev.advance(dt)
+    w = dev.download()
+    dev.close()
+    ne = problems.var_index(prob.ndir)["nevolve"]
+    assert_close_state(prob, w, ref, 1e-12, ne, f"{ndim}-D {prob.nx} {codes}")
