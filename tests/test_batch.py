@@ -94,6 +94,8 @@ def test_ensemble_first_steps_match_oracle(mods, oracle_mod):
         dts = [o.step(prob.courantpar) for _ in range(it[i])]
         assert t[i] == pytest.approx(sum(dts), rel=1e-8)
         errs = per_var_errors(prob, ws[i], o.get_w(), 8)
+        print(f"tube {i}: rel. dt-sum difference {abs(t[i] - sum(dts)) / sum(dts):.1e}, "
+              "states rel-L2 " + " ".join(f"{e:.1e}" for e in errs))
         assert max(errs) <= 1e-8, (i, errs)
 
 
