@@ -803,9 +803,11 @@ __device__ __forceinline__ void final_cells(const FinalArgs& a, const GridDesc& 
       }
     }
   }
-  // register update and cmax of the (stored) result
-  S* __restrict__ out = (S*)a.out;
-  const S* __restrict__ base = (const S*)a.base;
+  // register update and cmax of the (stored) result.  out may be base (the
+  // in-place stages): each element is read by the thread that writes it,
+  // before it writes it, so the two pointers carry no no-alias promise.
+  S* out = (S*)a.out;
+  const S* base = (const S*)a.base;
   S o[V][NE];
   // RK4's last stage: the reference's w1 = (w1 + 2*w2 + w3 - 4*w)/3 of the
   // three earlier stage registers and the step's base w, added to the update
